@@ -1,0 +1,158 @@
+/*
+ * dns_hip.h -- C ABI of the MI355X (gfx950) 2D pseudo-spectral vorticity solver.
+ *
+ * This is the boundary the host driver (mem_hip), the Python mirror
+ * (optimizec_amd) and any FFI caller bind.  Plain C types only: no HIP, torch or
+ * C++ types cross it.  Every call returns a dns_status; the text of the last error
+ * of a handle is available from dns_last_error().  A handle is single-threaded (no
+ * internal locking); all device work of a handle is queued on one HIP stream and
+ * calls that hand results to the host synchronize that stream.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * mannetroll/optimizec, directory cuda/).
+ *
+ * Data layouts on the host side of the ABI follow the reference:
+ *   om2 / fnm1 : [iz][ix], N rows x N/2 complex (interleaved re, im doubles),
+ *                kz = iz for iz < N/2, iz - N otherwise; kx = ix   (cuda_dns.h:75-76)
+ *   physical planes : [z][x], M x M doubles, M = 3N/2            (cuda_dns.h:33-34)
+ */
+#ifndef DNS_HIP_H
+#define DNS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dns_handle dns_handle;
+
+typedef enum {
+    DNS_OK = 0,
+    DNS_E_ARG = 1,          /* invalid argument or size */
+    DNS_E_UNSUPPORTED = 2,  /* grid size or option not built into this library */
+    DNS_E_HIP = 3,          /* HIP runtime error (message in dns_last_error) */
+    DNS_E_NOMEM = 4,        /* device or host allocation failed */
+    DNS_E_STATE = 5,        /* call out of order (e.g. step before an initial condition) */
+    DNS_E_IO = 6,           /* file could not be opened / read / written */
+    DNS_E_FORMAT = 7        /* restart file rejected (magic, version, sizes, dims, counts) */
+} dns_status;
+
+typedef enum {
+    DNS_IC_PAO_KM3 = 0,  /* the reference PAO-k^-3 spectrum (pao_km3_cuda.cu:59-498) */
+    DNS_IC_K4EXP = 1     /* seeded random phases, E(k) ~ (k/K0)^4 exp(-2 (k/K0)^2), unit energy */
+} dns_ic_kind;
+
+typedef enum {
+    DNS_PREC_FP64 = 0
+} dns_precision;
+
+typedef enum {
+    DNS_FIELD_U = 0,        /* physical u (UR_full comp 0)                     */
+    DNS_FIELD_W = 1,        /* physical w (UR_full comp 1)                     */
+    DNS_FIELD_OMEGA = 2,    /* physical vorticity (OM2PHYS, om2phys_cuda.cu:153) */
+    DNS_FIELD_PSI = 3,      /* stream function (dnsCudaStreamFunc, om2phys_cuda.cu:258) */
+    DNS_FIELD_KINETIC = 4   /* |u| (dnsCudaKinetic, om2phys_cuda.cu:358)        */
+} dns_field;
+
+/* Configuration of dnsCudaCreate(S, N, Re, K0) (cuda_dns.h:108) plus the new knobs. */
+typedef struct {
+    int    N;          /* base grid, multiple of 4; M = 3N/2 physical points per direction */
+    double Re;         /* Reynolds number, visc = 1/Re (pao_km3_cuda.cu:380-382)            */
+    double K0;         /* spectrum peak wavenumber                                          */
+    double cflnum;     /* CFL number (mem_cuda.cu:1755, S.cflnum = CFL)                     */
+    int    precision;  /* dns_precision                                                     */
+    int    device;     /* HIP device ordinal                                                */
+} dns_config;
+
+/* The scalar part of DnsDeviceState (cuda_dns.h:39-69), in FP64. */
+typedef struct {
+    int    N, M, H, NK;
+    int    it, ifn;
+    double Re, K0, visc, t, dt, cn, cnm1, cflnum, cflm;
+} dns_scalars;
+
+/* ---- life cycle ------------------------------------------------------ */
+/* Replaces dnsCudaCreate (cuda_dns.h:108; dns_cuda.cu:204-391): allocates the
+ * state and every work buffer on the device.  Scalars start as mem_cuda.cu:1760-1765
+ * (it=0, dt=0, cn=1, cnm1=0, ifn=1). */
+dns_status dns_create(const dns_config *cfg, dns_handle **out);
+/* Replaces dnsCudaDestroy (cuda_dns.h:109; dns_cuda.cu:393-423). */
+void dns_destroy(dns_handle *h);
+/* Text of the last error on h (or of the last failed dns_create when h is NULL). */
+const char *dns_last_error(const dns_handle *h);
+/* 1 if this library was built with kernels for base grid N, else 0. */
+int dns_supported_n(int N);
+
+/* ---- initial condition ------------------------------------------------ */
+/* Replaces dnsCudaInit = PAO + CALCOM (cuda_dns.h:123; dns_cuda.cu:591-607): sets
+ * visc, om2 (and fnm1 = 0), and prepares the inverse transforms of om2.  For
+ * DNS_IC_PAO_KM3 the printed PAO block (pao_km3_cuda.cu:405-424) is written into
+ * log (may be NULL).  seed is used by DNS_IC_K4EXP only. */
+dns_status dns_init_ic(dns_handle *h, int kind, uint64_t seed, char *log, int logcap);
+
+/* ---- time stepping ------------------------------------------------------ */
+/* Initial NEXTDT after the first STEP2A (mem_cuda.cu:1784-1788; dns_cuda.cu:562-570):
+ * dt = cflnum / CFLM / pi.  Requires it == 0 and dt == 0 (returns DNS_E_STATE otherwise). */
+dns_status dns_start(dns_handle *h);
+/* nsteps iterations of the mem_cuda loop body (mem_cuda.cu:1848-1857): it += 1,
+ * STEP2B -> STEP3 -> STEP2A (here: row pass, then fused column pass), t += dt_old,
+ * cnm1 = cn.  Does not call NEXTDT; asynchronous on the handle's stream. */
+dns_status dns_step(dns_handle *h, int nsteps);
+/* Replaces next_dt_gpu (dns_cuda.cu:553-580): CFLM = max |u|/DX + |w|/DZ over the
+ * M x M grid of the current state; it==0 && dt==0: dt = cflnum/CFLM/pi; otherwise
+ * CFL = CFLM dt pi, cn = 0.8 + 0.2 cflnum/CFL, dt *= cn.  Synchronizes. */
+dns_status dns_next_dt(dns_handle *h);
+
+/* ---- diagnostics -------------------------------------------------------- */
+/* E = sum m |om2|^2 / K^2, Z = sum m |om2|^2 (m = 1 for kx = 0, else 2); the
+ * reference energy convention Q2 (pao_km3_cuda.cu:366-378).  Synchronizes. */
+dns_status dns_energy_enstrophy(dns_handle *h, double *E, double *Z);
+/* OM2PHYS + computeDisplaySigma (mem_cuda.cu:820-870): std-dev of the 8-bit
+ * quantized physical vorticity.  Synchronizes. */
+dns_status dns_display_sigma(dns_handle *h, double *sigma);
+/* Physical plane [z][x] (M x M doubles) of a dns_field into host memory. */
+dns_status dns_fetch_field(dns_handle *h, int field, double *host);
+
+/* ---- state access -------------------------------------------------------- */
+dns_status dns_get_scalars(const dns_handle *h, dns_scalars *out);
+/* Sets it, ifn, Re, K0, visc, t, dt, cn, cnm1, cflnum, cflm (N, M, H, NK ignored). */
+dns_status dns_set_scalars(dns_handle *h, const dns_scalars *in);
+dns_status dns_get_om2(dns_handle *h, double *host);
+dns_status dns_set_om2(dns_handle *h, const double *host);
+dns_status dns_get_fnm1(dns_handle *h, double *host);
+dns_status dns_set_fnm1(dns_handle *h, const double *host);
+
+/* ---- restart.bin (mem_cuda.cu:1019-1394) ---------------------------------- */
+/* Writes RestartHeader v2 with real_bytes=8, cplx_bytes=16, then UR_full (u, w,
+ * omega physical, 3 x M x M), om2 and fnm1. */
+dns_status dns_save_restart(dns_handle *h, const char *path);
+/* Reads v1 or v2 files with 4- or 8-byte reals (FP32 files are upcast); rejects bad
+ * magic, version, header size, real/cplx size, dims or counts with DNS_E_FORMAT. */
+dns_status dns_load_restart(dns_handle *h, const char *path);
+
+/* ---- host-only helpers (no device needed) ---------------------------------- */
+/* PAO-k^-3 spectrum as om2 [N][N/2] complex (after CALCOM); returns the final LCG
+ * seed (the printed "PAO seed"), or -1 on bad arguments. */
+int dns_ic_pao_host(int N, double Re, double K0, double *om2, char *log, int logcap, double *visc);
+/* k^4 exp initial condition as om2 [N][N/2] complex. */
+dns_status dns_ic_k4exp_host(int N, double K0, uint64_t seed, double *om2);
+
+/* ---- measurement ---------------------------------------------------------- */
+/* Enables HIP-event timing of every hot-kernel launch on the handle's stream. */
+dns_status dns_set_kernel_timing(dns_handle *h, int enabled);
+/* Per-kernel totals since the last reset: names[i] (static strings), launches,
+ * total milliseconds, algorithmic bytes per launch.  *n is capacity in, count out. */
+dns_status dns_kernel_timing(dns_handle *h, const char **names, int64_t *launches,
+                             double *total_ms, double *bytes_per_launch, int *n);
+dns_status dns_reset_kernel_timing(dns_handle *h);
+/* Blocks until all queued work of the handle is done. */
+dns_status dns_synchronize(dns_handle *h);
+/* Device bytes allocated by the handle. */
+size_t dns_device_bytes(const dns_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DNS_HIP_H */

@@ -1,0 +1,266 @@
+"""optimizec_amd -- MI355X-native 2D pseudo-spectral vorticity DNS (the mem_cuda time step).
+
+Python mirror of the C ABI in ``include/dns_hip.h`` (ctypes over the in-tree
+``libdns_hip.so``).  The solver state lives on the GPU; this module only moves host
+arrays across the boundary.  There is no CPU fallback: if the library or a HIP
+device is missing, ``Solver`` raises.
+
+Names follow the reference driver (cuda/cuda_dns.h, cuda/mem_cuda.cu): ``om2`` and
+``fnm1`` are the spectral vorticity and the previous nonlinear term, [iz][ix] with
+N rows of N/2 complex values; physical planes are M x M with M = 3N/2.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Solver", "Scalars", "DnsError", "library_path", "load_library", "ic_pao_host",
+           "ic_k4exp_host", "supported_n", "FIELD_U", "FIELD_W", "FIELD_OMEGA", "FIELD_PSI",
+           "FIELD_KINETIC", "IC_PAO_KM3", "IC_K4EXP"]
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+IC_PAO_KM3, IC_K4EXP = 0, 1
+FIELD_U, FIELD_W, FIELD_OMEGA, FIELD_PSI, FIELD_KINETIC = 0, 1, 2, 3, 4
+_STATUS = {0: "DNS_OK", 1: "DNS_E_ARG", 2: "DNS_E_UNSUPPORTED", 3: "DNS_E_HIP", 4: "DNS_E_NOMEM",
+           5: "DNS_E_STATE", 6: "DNS_E_IO", 7: "DNS_E_FORMAT"}
+
+
+class DnsError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("Re", ctypes.c_double), ("K0", ctypes.c_double),
+                ("cflnum", ctypes.c_double), ("precision", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+class Scalars(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_int), ("M", ctypes.c_int), ("H", ctypes.c_int), ("NK", ctypes.c_int),
+        ("it", ctypes.c_int), ("ifn", ctypes.c_int),
+        ("Re", ctypes.c_double), ("K0", ctypes.c_double), ("visc", ctypes.c_double),
+        ("t", ctypes.c_double), ("dt", ctypes.c_double), ("cn", ctypes.c_double),
+        ("cnm1", ctypes.c_double), ("cflnum", ctypes.c_double), ("cflm", ctypes.c_double),
+    ]
+
+
+_lib = None
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+
+# every symbol include/dns_hip.h declares, with its ctypes signature
+SIGNATURES = {
+    "dns_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_P)]),
+    "dns_destroy": (None, [_P]),
+    "dns_last_error": (ctypes.c_char_p, [_P]),
+    "dns_supported_n": (ctypes.c_int, [ctypes.c_int]),
+    "dns_init_ic": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_int]),
+    "dns_start": (ctypes.c_int, [_P]),
+    "dns_step": (ctypes.c_int, [_P, ctypes.c_int]),
+    "dns_next_dt": (ctypes.c_int, [_P]),
+    "dns_energy_enstrophy": (ctypes.c_int, [_P, _D, _D]),
+    "dns_display_sigma": (ctypes.c_int, [_P, _D]),
+    "dns_fetch_field": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "dns_get_scalars": (ctypes.c_int, [_P, ctypes.POINTER(Scalars)]),
+    "dns_set_scalars": (ctypes.c_int, [_P, ctypes.POINTER(Scalars)]),
+    "dns_get_om2": (ctypes.c_int, [_P, _D]),
+    "dns_set_om2": (ctypes.c_int, [_P, _D]),
+    "dns_get_fnm1": (ctypes.c_int, [_P, _D]),
+    "dns_set_fnm1": (ctypes.c_int, [_P, _D]),
+    "dns_save_restart": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "dns_load_restart": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "dns_ic_pao_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D,
+                                       ctypes.c_char_p, ctypes.c_int, _D]),
+    "dns_ic_k4exp_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _D]),
+    "dns_set_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "dns_kernel_timing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_char_p),
+                                         ctypes.POINTER(ctypes.c_int64), _D, _D,
+                                         ctypes.POINTER(ctypes.c_int)]),
+    "dns_reset_kernel_timing": (ctypes.c_int, [_P]),
+    "dns_synchronize": (ctypes.c_int, [_P]),
+    "dns_device_bytes": (ctypes.c_size_t, [_P]),
+}
+
+
+def library_path() -> str:
+    return os.path.join(_PKG, "libdns_hip.so")
+
+
+def load_library():
+    """Loads the in-tree libdns_hip.so (built by optimizec_amd/build.py); raises if absent."""
+    global _lib
+    if _lib is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise DnsError(3, f"{path} is not built (run python -m optimizec_amd.build)")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+def supported_n(N: int) -> bool:
+    return bool(load_library().dns_supported_n(N))
+
+
+def ic_pao_host(N: int, Re: float, K0: float):
+    """PAO-k^-3 initial om2 computed on the host: (om2, seed, printed block, visc)."""
+    om = np.zeros((N, N // 2), dtype=np.complex128)
+    buf = ctypes.create_string_buffer(8192)
+    visc = ctypes.c_double()
+    seed = load_library().dns_ic_pao_host(N, Re, K0, _ptr(om), buf, len(buf), ctypes.byref(visc))
+    if seed < 0:
+        raise DnsError(1, "dns_ic_pao_host rejected its arguments")
+    return om, seed, buf.value.decode(), visc.value
+
+
+def ic_k4exp_host(N: int, K0: float, seed: int) -> np.ndarray:
+    om = np.zeros((N, N // 2), dtype=np.complex128)
+    st = load_library().dns_ic_k4exp_host(N, K0, ctypes.c_uint64(seed), _ptr(om))
+    if st:
+        raise DnsError(st, "dns_ic_k4exp_host rejected its arguments")
+    return om
+
+
+class Solver:
+    """One device state: the C handle of dns_create (dnsCudaCreate, cuda_dns.h:108)."""
+
+    def __init__(self, N: int, Re: float, K0: float, cflnum: float, device: int = 0):
+        L = load_library()
+        cfg = Config(N, float(Re), float(K0), float(cflnum), 0, int(device))
+        h = _P()
+        st = L.dns_create(ctypes.byref(cfg), ctypes.byref(h))
+        if st:
+            raise DnsError(st, L.dns_last_error(None).decode())
+        self._h = h
+        self.N, self.M, self.H = N, 3 * N // 2, N // 2
+
+    def _check(self, st: int) -> None:
+        if st:
+            raise DnsError(st, load_library().dns_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load_library().dns_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- initial condition and stepping (dnsCudaInit, next_dt_gpu, loop body)
+    def init_ic(self, kind: int = IC_K4EXP, seed: int = 0) -> str:
+        buf = ctypes.create_string_buffer(8192)
+        self._check(load_library().dns_init_ic(self._h, kind, ctypes.c_uint64(seed), buf, len(buf)))
+        return buf.value.decode()
+
+    def start(self) -> None:
+        self._check(load_library().dns_start(self._h))
+
+    def step(self, n: int = 1) -> None:
+        self._check(load_library().dns_step(self._h, n))
+
+    def next_dt(self) -> None:
+        self._check(load_library().dns_next_dt(self._h))
+
+    def synchronize(self) -> None:
+        self._check(load_library().dns_synchronize(self._h))
+
+    def run(self, steps: int, update: int):
+        """mem_cuda's loop (mem_cuda.cu:1848-1921) from the current it to `steps`; returns
+        rows (it, t, dt, cn, cflm, E, Z) for the diagnostic steps."""
+        rows = []
+        s = self.scalars()
+        for it in range(s.it + 1, steps + 1):
+            self.step(1)
+            if it == 1 or it % update == 0 or it == steps:
+                self.next_dt()
+                sc = self.scalars()
+                E, Z = self.energy_enstrophy()
+                rows.append((it, sc.t, sc.dt, sc.cn, sc.cflm, E, Z))
+        return np.array(rows, dtype=np.float64).reshape(-1, 7)
+
+    # -- diagnostics
+    def energy_enstrophy(self):
+        e, z = ctypes.c_double(), ctypes.c_double()
+        self._check(load_library().dns_energy_enstrophy(self._h, ctypes.byref(e), ctypes.byref(z)))
+        return e.value, z.value
+
+    def display_sigma(self) -> float:
+        s = ctypes.c_double()
+        self._check(load_library().dns_display_sigma(self._h, ctypes.byref(s)))
+        return s.value
+
+    def field(self, which: int) -> np.ndarray:
+        out = np.empty((self.M, self.M), dtype=np.float64)
+        self._check(load_library().dns_fetch_field(self._h, which, _ptr(out)))
+        return out
+
+    # -- state access
+    def scalars(self) -> Scalars:
+        s = Scalars()
+        self._check(load_library().dns_get_scalars(self._h, ctypes.byref(s)))
+        return s
+
+    def set_scalars(self, s: Scalars) -> None:
+        self._check(load_library().dns_set_scalars(self._h, ctypes.byref(s)))
+
+    def om2(self) -> np.ndarray:
+        out = np.empty((self.N, self.H), dtype=np.complex128)
+        self._check(load_library().dns_get_om2(self._h, _ptr(out)))
+        return out
+
+    def set_om2(self, a) -> None:
+        a = np.ascontiguousarray(a, dtype=np.complex128).reshape(self.N, self.H)
+        self._check(load_library().dns_set_om2(self._h, _ptr(a)))
+
+    def fnm1(self) -> np.ndarray:
+        out = np.empty((self.N, self.H), dtype=np.complex128)
+        self._check(load_library().dns_get_fnm1(self._h, _ptr(out)))
+        return out
+
+    def set_fnm1(self, a) -> None:
+        a = np.ascontiguousarray(a, dtype=np.complex128).reshape(self.N, self.H)
+        self._check(load_library().dns_set_fnm1(self._h, _ptr(a)))
+
+    def save_restart(self, path: str) -> None:
+        self._check(load_library().dns_save_restart(self._h, path.encode()))
+
+    def load_restart(self, path: str) -> None:
+        self._check(load_library().dns_load_restart(self._h, path.encode()))
+
+    # -- measurement
+    def set_kernel_timing(self, on: bool) -> None:
+        self._check(load_library().dns_set_kernel_timing(self._h, 1 if on else 0))
+
+    def reset_kernel_timing(self) -> None:
+        self._check(load_library().dns_reset_kernel_timing(self._h))
+
+    def kernel_timing(self) -> dict:
+        n = ctypes.c_int(8)
+        names = (ctypes.c_char_p * 8)()
+        launches = (ctypes.c_int64 * 8)()
+        ms = (ctypes.c_double * 8)()
+        bytes_ = (ctypes.c_double * 8)()
+        self._check(load_library().dns_kernel_timing(self._h, names, launches, ms, bytes_, ctypes.byref(n)))
+        return {names[i].decode(): {"launches": launches[i], "total_ms": ms[i], "bytes_per_launch": bytes_[i]}
+                for i in range(n.value)}
+
+    def device_bytes(self) -> int:
+        return int(load_library().dns_device_bytes(self._h))

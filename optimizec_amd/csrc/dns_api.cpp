@@ -1,0 +1,840 @@
+// dns_api.cpp -- host side of the C ABI (include/dns_hip.h): state, stream, launch
+// sequencing, the reference's dt/CFL logic, diagnostics and restart.bin.
+//
+// The call order and the scalar logic follow the reference driver:
+//   init          mem_cuda.cu:1743-1789 (create, scalar reset, PAO + CALCOM, initial
+//                 STEP2A, NEXTDT at it=0)
+//   loop body     mem_cuda.cu:1848-1857 (it++, dt_old = dt, STEP2B, STEP3, STEP2A,
+//                 t += dt_old; STEP3 ends with cnm1 = cn, step3_cuda.cu:172)
+//   NEXTDT        dns_cuda.cu:553-580
+//   restart.bin   mem_cuda.cu:1019-1394
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cerrno>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dns_hip.h"
+#include "dns_kernels.h"
+
+using dnsfft::cd;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct TimedLaunch {
+    int kernel;
+    hipEvent_t a, b;
+};
+
+enum { KT_ROW = 0, KT_COL = 1, KT_ROW_CFL = 2, KT_COUNT = 3 };
+const char *const kKernelNames[KT_COUNT] = {"k_row", "k_col", "k_row_cfl"};
+
+}  // namespace
+
+struct dns_handle {
+    int N = 0, M = 0, H = 0, NK = 0;
+    int device = 0;
+    dns_scalars sc{};
+    hipStream_t stream = nullptr;
+    cd *om = nullptr, *fn = nullptr, *V = nullptr, *G = nullptr, *tw = nullptr;
+    cd *Vd = nullptr;           // diagnostics: 2 z-inverted fields (lazy)
+    double *phys = nullptr;     // diagnostics: one M x M physical plane (lazy)
+    double *cflrow = nullptr;   // per-row CFL maxima (M)
+    double *partEZ = nullptr;   // per-column E, Z partials (2 H)
+    double *red = nullptr;      // min/max partials
+    unsigned long long *redu = nullptr;  // pixel partials
+    size_t bytes = 0;
+    bool have_ic = false;
+    bool v_valid = false;
+    std::string err;
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    int64_t launches[KT_COUNT] = {0, 0, 0};
+    double total_ms[KT_COUNT] = {0, 0, 0};
+};
+
+namespace {
+
+constexpr int kRedBlocks = 1024;
+
+dns_status fail(dns_handle *h, dns_status st, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h)
+        h->err = buf;
+    else
+        g_create_error = buf;
+    return st;
+}
+
+#define HIPCHK(h, expr)                                                                            \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return fail((h), DNS_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+dns_status dev_alloc(dns_handle *h, void **p, size_t bytes, const char *name)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(h, DNS_E_NOMEM, "hipMalloc(%s, %.3f GiB) failed: %s", name, bytes / 1073741824.0,
+                    hipGetErrorString(e));
+    }
+    h->bytes += bytes;
+    return DNS_OK;
+}
+
+hipEvent_t take_event(dns_handle *h)
+{
+    if (!h->event_pool.empty()) {
+        hipEvent_t e = h->event_pool.back();
+        h->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// resolve recorded launches into the per-kernel totals (synchronizes the stream)
+void drain_timing(dns_handle *h)
+{
+    if (h->pending.empty()) return;
+    (void)hipStreamSynchronize(h->stream);
+    for (TimedLaunch &t : h->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            h->total_ms[t.kernel] += ms;
+            h->launches[t.kernel] += 1;
+        }
+        h->event_pool.push_back(t.a);
+        h->event_pool.push_back(t.b);
+    }
+    h->pending.clear();
+}
+
+dns_status launch(dns_handle *h, int timed_kernel, LaunchReq &r)
+{
+    r.stream = h->stream;
+    TimedLaunch tl{timed_kernel, nullptr, nullptr};
+    if (h->timing && timed_kernel >= 0) {
+        if (h->pending.size() > 4096) drain_timing(h);
+        tl.a = take_event(h);
+        tl.b = take_event(h);
+        if (tl.a && tl.b) (void)hipEventRecord(tl.a, h->stream);
+    }
+    hipError_t e = dns_kernels_launch(h->H, r);
+    if (e != hipSuccess) return fail(h, DNS_E_HIP, "kernel launch (%d) failed: %s", r.what, hipGetErrorString(e));
+    if (tl.a && tl.b) {
+        (void)hipEventRecord(tl.b, h->stream);
+        h->pending.push_back(tl);
+    }
+    return DNS_OK;
+}
+
+ColArgs inverse_args(dns_handle *h, int nf, const int *fields, size_t vstride)
+{
+    ColArgs a{};
+    a.do_forward = 0;
+    a.nfields = nf;
+    for (int i = 0; i < nf; ++i) a.fields[i] = fields[i];
+    a.vstride = vstride;
+    a.divxz = 1.0 / (double(h->M) * double(h->M));
+    return a;
+}
+
+// z-inverse of the step's four spectra (u, w, d_x omega, d_z omega) of the current om
+dns_status refresh_v(dns_handle *h)
+{
+    static const int f4[4] = {DNS_F_U, DNS_F_W, DNS_F_OX, DNS_F_OZ};
+    LaunchReq r{};
+    r.what = LAUNCH_COL;
+    r.om = h->om;
+    r.fn = h->fn;
+    r.G = h->G;
+    r.V = h->V;
+    r.tw = h->tw;
+    r.col = inverse_args(h, 4, f4, (size_t)h->M * h->H);
+    dns_status st = launch(h, -1, r);
+    if (st == DNS_OK) h->v_valid = true;
+    return st;
+}
+
+dns_status ensure_diag_buffers(dns_handle *h)
+{
+    if (!h->Vd) {
+        dns_status st = dev_alloc(h, (void **)&h->Vd, sizeof(cd) * 2 * (size_t)h->M * h->H, "diag spectra");
+        if (st != DNS_OK) return st;
+    }
+    if (!h->phys) {
+        dns_status st = dev_alloc(h, (void **)&h->phys, sizeof(double) * (size_t)h->M * h->M, "physical plane");
+        if (st != DNS_OK) return st;
+    }
+    return DNS_OK;
+}
+
+// physical plane of a dns_field into h->phys
+dns_status compute_field(dns_handle *h, int field)
+{
+    dns_status st = ensure_diag_buffers(h);
+    if (st != DNS_OK) return st;
+    int fields[2];
+    int nf = 1;
+    switch (field) {
+        case DNS_FIELD_U: fields[0] = DNS_F_U; break;
+        case DNS_FIELD_W: fields[0] = DNS_F_W; break;
+        case DNS_FIELD_OMEGA: fields[0] = DNS_F_OMEGA; break;
+        case DNS_FIELD_PSI: fields[0] = DNS_F_PSI; break;
+        case DNS_FIELD_KINETIC:
+            fields[0] = DNS_F_U;
+            fields[1] = DNS_F_W;
+            nf = 2;
+            break;
+        default: return fail(h, DNS_E_ARG, "unknown field %d", field);
+    }
+    const size_t vs = (size_t)h->M * h->H;
+    LaunchReq r{};
+    r.what = LAUNCH_COL;
+    r.om = h->om;
+    r.fn = h->fn;
+    r.G = h->G;
+    r.V = h->Vd;
+    r.tw = h->tw;
+    r.col = inverse_args(h, nf, fields, vs);
+    st = launch(h, -1, r);
+    if (st != DNS_OK) return st;
+    LaunchReq p{};
+    p.what = LAUNCH_ROW_PHYS;
+    p.tw = h->tw;
+    p.Va = h->Vd;
+    p.Vb = (nf == 2) ? h->Vd + vs : nullptr;
+    p.out0 = h->phys;
+    p.out1 = nullptr;
+    p.out_mode = (field == DNS_FIELD_KINETIC) ? 1 : 0;
+    return launch(h, -1, p);
+}
+
+dns_status compute_cflm(dns_handle *h, double *cflm)
+{
+    if (!h->v_valid) {
+        dns_status st = refresh_v(h);
+        if (st != DNS_OK) return st;
+    }
+    const double PI = 3.14159265358979323846;
+    LaunchReq r{};
+    r.what = LAUNCH_ROW_CFL;
+    r.V = h->V;
+    r.vstride = (size_t)h->M * h->H;
+    r.Gw = nullptr;
+    r.tw = h->tw;
+    r.cflrow = h->cflrow;
+    r.DX = 2.0 * PI / double(h->N);
+    r.DZ = 2.0 * PI / double(h->N);
+    dns_status st = launch(h, KT_ROW_CFL, r);
+    if (st != DNS_OK) return st;
+    std::vector<double> rows(h->M);
+    HIPCHK(h, hipMemcpyAsync(rows.data(), h->cflrow, sizeof(double) * h->M, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double m = 0.0;
+    for (double v : rows) m = v > m ? v : m;
+    *cflm = m;
+    return DNS_OK;
+}
+
+// host [iz][ix] <-> device [ix][iz] through the G buffer as staging
+dns_status upload_spec(dns_handle *h, const double *host, cd *dst)
+{
+    const size_t n = (size_t)h->N * h->H;
+    HIPCHK(h, hipMemcpyAsync(h->G, host, sizeof(cd) * n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, dns_launch_transpose(h->G, dst, h->N, h->H, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DNS_OK;
+}
+
+dns_status download_spec(dns_handle *h, const cd *src, double *host)
+{
+    const size_t n = (size_t)h->N * h->H;
+    HIPCHK(h, dns_launch_transpose(src, h->G, h->H, h->N, h->stream));
+    HIPCHK(h, hipMemcpyAsync(host, h->G, sizeof(cd) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DNS_OK;
+}
+
+void reset_scalars(dns_handle *h)
+{
+    // mem_cuda.cu:1755-1765
+    h->sc.it = 0;
+    h->sc.ifn = 1;
+    h->sc.t = 0.0;
+    h->sc.dt = 0.0;
+    h->sc.cn = 1.0;
+    h->sc.cnm1 = 0.0;
+    h->sc.cflm = 0.0;
+}
+
+}  // namespace
+
+// =====================================================================================
+// life cycle
+// =====================================================================================
+extern "C" int dns_supported_n(int N)
+{
+    return (N >= 16 && N % 4 == 0 && dns_kernels_supported(N / 2)) ? 1 : 0;
+}
+
+extern "C" const char *dns_last_error(const dns_handle *h)
+{
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
+{
+    if (!cfg || !out) return fail(nullptr, DNS_E_ARG, "dns_create: null argument");
+    *out = nullptr;
+    if (cfg->N < 4 || cfg->N % 4 != 0) return fail(nullptr, DNS_E_ARG, "N=%d must be a positive multiple of 4", cfg->N);
+    if (!dns_supported_n(cfg->N))
+        return fail(nullptr, DNS_E_UNSUPPORTED, "N=%d: no kernels built for H=N/2=%d", cfg->N, cfg->N / 2);
+    if (cfg->precision != DNS_PREC_FP64)
+        return fail(nullptr, DNS_E_UNSUPPORTED, "precision %d is not built", cfg->precision);
+    if (!(cfg->Re > 0.0)) return fail(nullptr, DNS_E_ARG, "Re must be positive");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0)
+        return fail(nullptr, DNS_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(nullptr, DNS_E_ARG, "device %d out of range", cfg->device);
+
+    dns_handle *h = new dns_handle();
+    h->N = cfg->N;
+    h->H = cfg->N / 2;
+    h->M = 3 * cfg->N / 2;
+    h->NK = 3 * cfg->N / 4 + 1;
+    h->device = cfg->device;
+    h->sc.N = h->N;
+    h->sc.M = h->M;
+    h->sc.H = h->H;
+    h->sc.NK = h->NK;
+    h->sc.Re = cfg->Re;
+    h->sc.K0 = cfg->K0;
+    h->sc.visc = 0.0;
+    h->sc.cflnum = cfg->cflnum;
+    reset_scalars(h);
+
+    auto bail = [&](dns_status st) {
+        g_create_error = h->err;
+        dns_destroy(h);
+        return st;
+    };
+    if ((e = hipSetDevice(h->device)) != hipSuccess)
+        return bail(fail(h, DNS_E_HIP, "hipSetDevice: %s", hipGetErrorString(e)));
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(fail(h, DNS_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
+
+    const size_t nspec = (size_t)h->N * h->H;
+    const size_t nmid = (size_t)h->M * h->H;
+    dns_status st;
+    if ((st = dev_alloc(h, (void **)&h->om, sizeof(cd) * nspec, "om2")) != DNS_OK) return bail(st);
+    if ((st = dev_alloc(h, (void **)&h->fn, sizeof(cd) * nspec, "fnm1")) != DNS_OK) return bail(st);
+    if ((st = dev_alloc(h, (void **)&h->V, sizeof(cd) * 4 * nmid, "z-inverse fields")) != DNS_OK) return bail(st);
+    if ((st = dev_alloc(h, (void **)&h->G, sizeof(cd) * nmid, "x-forward product")) != DNS_OK) return bail(st);
+    if ((st = dev_alloc(h, (void **)&h->tw, sizeof(cd) * h->M, "twiddles")) != DNS_OK) return bail(st);
+    if ((st = dev_alloc(h, (void **)&h->cflrow, sizeof(double) * h->M, "cfl rows")) != DNS_OK) return bail(st);
+    if ((st = dev_alloc(h, (void **)&h->partEZ, sizeof(double) * 2 * h->H, "energy partials")) != DNS_OK)
+        return bail(st);
+    if ((st = dev_alloc(h, (void **)&h->red, sizeof(double) * 2 * kRedBlocks, "minmax partials")) != DNS_OK)
+        return bail(st);
+    if ((st = dev_alloc(h, (void **)&h->redu, sizeof(unsigned long long) * 2 * kRedBlocks, "pixel partials")) !=
+        DNS_OK)
+        return bail(st);
+
+    // W_M^t = e^{-2 pi i t / M}, evaluated in long double and rounded once
+    std::vector<cd> tw(h->M);
+    for (int t = 0; t < h->M; ++t) {
+        const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)h->M;
+        tw[t] = cd{(double)cosl(a), -(double)sinl(a)};
+    }
+    if ((e = hipMemcpy(h->tw, tw.data(), sizeof(cd) * h->M, hipMemcpyHostToDevice)) != hipSuccess)
+        return bail(fail(h, DNS_E_HIP, "twiddle upload: %s", hipGetErrorString(e)));
+    if ((e = hipMemset(h->om, 0, sizeof(cd) * nspec)) != hipSuccess ||
+        (e = hipMemset(h->fn, 0, sizeof(cd) * nspec)) != hipSuccess)
+        return bail(fail(h, DNS_E_HIP, "state clear: %s", hipGetErrorString(e)));
+    *out = h;
+    return DNS_OK;
+}
+
+extern "C" void dns_destroy(dns_handle *h)
+{
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (TimedLaunch &t : h->pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (hipEvent_t ev : h->event_pool) (void)hipEventDestroy(ev);
+    void *ptrs[] = {h->om, h->fn, h->V, h->G, h->tw, h->Vd, h->phys, h->cflrow, h->partEZ, h->red, h->redu};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+// =====================================================================================
+// initial condition and stepping
+// =====================================================================================
+extern "C" dns_status dns_init_ic(dns_handle *h, int kind, uint64_t seed, char *log, int logcap)
+{
+    if (!h) return DNS_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    std::vector<double> om2((size_t)2 * h->N * h->H);
+    double visc = 1.0 / h->sc.Re;
+    if (kind == DNS_IC_PAO_KM3) {
+        if (dns_ic_pao_host(h->N, h->sc.Re, h->sc.K0, om2.data(), log, logcap, &visc) < 0)
+            return fail(h, DNS_E_ARG, "PAO initial condition rejected its arguments");
+    } else if (kind == DNS_IC_K4EXP) {
+        if (dns_ic_k4exp_host(h->N, h->sc.K0, seed, om2.data()) != DNS_OK)
+            return fail(h, DNS_E_ARG, "k4exp initial condition rejected its arguments (K0 > 0?)");
+        if (log && logcap > 0) log[0] = '\0';
+    } else {
+        return fail(h, DNS_E_ARG, "unknown initial condition kind %d", kind);
+    }
+    reset_scalars(h);
+    h->sc.visc = visc;
+    dns_status st = upload_spec(h, om2.data(), h->om);
+    if (st != DNS_OK) return st;
+    HIPCHK(h, hipMemsetAsync(h->fn, 0, sizeof(cd) * (size_t)h->N * h->H, h->stream));
+    h->have_ic = true;
+    return refresh_v(h);
+}
+
+extern "C" dns_status dns_start(dns_handle *h)
+{
+    if (!h) return DNS_E_ARG;
+    if (!h->have_ic) return fail(h, DNS_E_STATE, "dns_start before an initial condition");
+    if (!(h->sc.it == 0 && h->sc.dt == 0.0)) return fail(h, DNS_E_STATE, "dns_start needs it == 0 and dt == 0");
+    return dns_next_dt(h);
+}
+
+extern "C" dns_status dns_step(dns_handle *h, int nsteps)
+{
+    if (!h) return DNS_E_ARG;
+    if (!h->have_ic) return fail(h, DNS_E_STATE, "dns_step before an initial condition");
+    if (nsteps < 0) return fail(h, DNS_E_ARG, "nsteps < 0");
+    HIPCHK(h, hipSetDevice(h->device));
+    if (!h->v_valid) {
+        dns_status st = refresh_v(h);
+        if (st != DNS_OK) return st;
+    }
+    static const int f4[4] = {DNS_F_U, DNS_F_W, DNS_F_OX, DNS_F_OZ};
+    const size_t vs = (size_t)h->M * h->H;
+    const double PI = 3.14159265358979323846;
+    for (int s = 0; s < nsteps; ++s) {
+        h->sc.it += 1;
+        const double dt = h->sc.dt;
+        LaunchReq r{};
+        r.what = LAUNCH_ROW;
+        r.V = h->V;
+        r.vstride = vs;
+        r.Gw = h->G;
+        r.tw = h->tw;
+        r.cflrow = nullptr;
+        r.DX = r.DZ = 2.0 * PI / double(h->N);
+        dns_status st = launch(h, KT_ROW, r);
+        if (st != DNS_OK) return st;
+        LaunchReq c{};
+        c.what = LAUNCH_COL;
+        c.om = h->om;
+        c.fn = h->fn;
+        c.G = h->G;
+        c.V = h->V;
+        c.tw = h->tw;
+        c.col.do_forward = 1;
+        c.col.nfields = 4;
+        for (int i = 0; i < 4; ++i) c.col.fields[i] = f4[i];
+        c.col.vstride = vs;
+        c.col.divxz = 1.0 / (double(h->M) * double(h->M));
+        c.col.VT = 0.5 * h->sc.visc * dt;
+        c.col.c2 = 0.5 * dt * (2.0 + h->sc.cnm1);
+        c.col.c3 = -0.5 * dt * h->sc.cnm1;
+        st = launch(h, KT_COL, c);
+        if (st != DNS_OK) return st;
+        h->sc.t += dt;
+        h->sc.cnm1 = h->sc.cn;
+    }
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_next_dt(dns_handle *h)
+{
+    if (!h) return DNS_E_ARG;
+    if (!h->have_ic) return fail(h, DNS_E_STATE, "dns_next_dt before an initial condition");
+    HIPCHK(h, hipSetDevice(h->device));
+    const double PI = 3.14159265358979323846;
+    if (h->sc.it == 0 && h->sc.dt == 0.0) {
+        double cflm = 0.0;
+        dns_status st = compute_cflm(h, &cflm);
+        if (st != DNS_OK) return st;
+        h->sc.dt = h->sc.cflnum / cflm / PI;
+        h->sc.cflm = cflm;
+        return DNS_OK;
+    }
+    if (h->sc.ifn > 0 && (h->sc.it % h->sc.ifn) == 0) {
+        double cflm = 0.0;
+        dns_status st = compute_cflm(h, &cflm);
+        if (st != DNS_OK) return st;
+        const double cfl = cflm * h->sc.dt * PI;
+        h->sc.cn = 0.8 + 0.2 * (h->sc.cflnum / cfl);
+        h->sc.dt = h->sc.dt * h->sc.cn;
+        h->sc.cflm = cflm;
+    }
+    return DNS_OK;
+}
+
+// =====================================================================================
+// diagnostics
+// =====================================================================================
+extern "C" dns_status dns_energy_enstrophy(dns_handle *h, double *E, double *Z)
+{
+    if (!h || !E || !Z) return DNS_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, dns_launch_energy(h->om, h->N, h->H, h->partEZ, h->partEZ + h->H, h->stream));
+    std::vector<double> p(2 * (size_t)h->H);
+    HIPCHK(h, hipMemcpyAsync(p.data(), h->partEZ, sizeof(double) * p.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double e = 0.0, z = 0.0;
+    for (int i = 0; i < h->H; ++i) {
+        e += p[i];
+        z += p[h->H + i];
+    }
+    *E = e;
+    *Z = z;
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_display_sigma(dns_handle *h, double *sigma)
+{
+    if (!h || !sigma) return DNS_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    dns_status st = compute_field(h, DNS_FIELD_OMEGA);
+    if (st != DNS_OK) return st;
+    const size_t n = (size_t)h->M * h->M;
+    const int blocks = (int)std::min<size_t>(kRedBlocks, (n + 255) / 256);
+    HIPCHK(h, dns_launch_minmax(h->phys, n, h->red, blocks, h->stream));
+    std::vector<double> mm(2 * (size_t)blocks);
+    HIPCHK(h, hipMemcpyAsync(mm.data(), h->red, sizeof(double) * mm.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double mn = mm[0], mx = mm[1];
+    for (int b = 1; b < blocks; ++b) {
+        mn = std::min(mn, mm[2 * b]);
+        mx = std::max(mx, mm[2 * b + 1]);
+    }
+    const double rng = mx - mn;
+    const int is_const = std::fabs(rng) <= 1.0e-12 ? 1 : 0;
+    HIPCHK(h, dns_launch_pixels(h->phys, n, mn, rng, is_const, h->redu, blocks, h->stream));
+    std::vector<unsigned long long> ps(2 * (size_t)blocks);
+    HIPCHK(h, hipMemcpyAsync(ps.data(), h->redu, sizeof(unsigned long long) * ps.size(), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    unsigned long long s1 = 0, s2 = 0;
+    for (int b = 0; b < blocks; ++b) {
+        s1 += ps[2 * b];
+        s2 += ps[2 * b + 1];
+    }
+    const double mean = double(s1) / double(n);
+    const double var = std::max(0.0, double(s2) / double(n) - mean * mean);
+    *sigma = std::sqrt(var);
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_fetch_field(dns_handle *h, int field, double *host)
+{
+    if (!h || !host) return DNS_E_ARG;
+    if (!h->have_ic) return fail(h, DNS_E_STATE, "dns_fetch_field before an initial condition");
+    HIPCHK(h, hipSetDevice(h->device));
+    dns_status st = compute_field(h, field);
+    if (st != DNS_OK) return st;
+    HIPCHK(h, hipMemcpyAsync(host, h->phys, sizeof(double) * (size_t)h->M * h->M, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DNS_OK;
+}
+
+// =====================================================================================
+// state access
+// =====================================================================================
+extern "C" dns_status dns_get_scalars(const dns_handle *h, dns_scalars *out)
+{
+    if (!h || !out) return DNS_E_ARG;
+    *out = h->sc;
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_set_scalars(dns_handle *h, const dns_scalars *in)
+{
+    if (!h || !in) return DNS_E_ARG;
+    h->sc.it = in->it;
+    h->sc.ifn = in->ifn;
+    h->sc.Re = in->Re;
+    h->sc.K0 = in->K0;
+    h->sc.visc = in->visc;
+    h->sc.t = in->t;
+    h->sc.dt = in->dt;
+    h->sc.cn = in->cn;
+    h->sc.cnm1 = in->cnm1;
+    h->sc.cflnum = in->cflnum;
+    h->sc.cflm = in->cflm;
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_get_om2(dns_handle *h, double *host)
+{
+    if (!h || !host) return DNS_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    return download_spec(h, h->om, host);
+}
+
+extern "C" dns_status dns_set_om2(dns_handle *h, const double *host)
+{
+    if (!h || !host) return DNS_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    dns_status st = upload_spec(h, host, h->om);
+    if (st != DNS_OK) return st;
+    h->have_ic = true;
+    h->v_valid = false;
+    return refresh_v(h);
+}
+
+extern "C" dns_status dns_get_fnm1(dns_handle *h, double *host)
+{
+    if (!h || !host) return DNS_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    return download_spec(h, h->fn, host);
+}
+
+extern "C" dns_status dns_set_fnm1(dns_handle *h, const double *host)
+{
+    if (!h || !host) return DNS_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    return upload_spec(h, host, h->fn);
+}
+
+// =====================================================================================
+// restart.bin
+// =====================================================================================
+namespace {
+
+struct HeadInts {
+    char magic[16];
+    uint32_t version, header_bytes, real_bytes, cplx_bytes;
+    int32_t Nbase, N, NX, NZ, NK, NX_full, NZ_full, NK_full, it, ifn;
+};
+static_assert(sizeof(HeadInts) == 72, "restart header integer block is 72 bytes");
+
+constexpr size_t kScalarCount = 9;  // Re K0 visc t dt cn cnm1 cflnum cflm
+
+size_t align8(size_t x) { return (x + 7) & ~size_t(7); }
+// header sizes of RestartHeaderV1 / RestartHeader for a given real size
+size_t v1_bytes(size_t real_bytes) { return align8(72 + kScalarCount * real_bytes) + 24; }
+size_t v2_bytes(size_t real_bytes) { return v1_bytes(real_bytes) + 8; }
+
+bool write_all(FILE *fp, const void *p, size_t n) { return n == 0 || std::fwrite(p, 1, n, fp) == n; }
+bool read_all(FILE *fp, void *p, size_t n) { return n == 0 || std::fread(p, 1, n, fp) == n; }
+
+}  // namespace
+
+extern "C" dns_status dns_save_restart(dns_handle *h, const char *path)
+{
+    if (!h || !path) return DNS_E_ARG;
+    if (!h->have_ic) return fail(h, DNS_E_STATE, "dns_save_restart before an initial condition");
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t hb = v2_bytes(8);
+    std::vector<unsigned char> head(hb, 0);
+    HeadInts hi{};
+    std::snprintf(hi.magic, sizeof hi.magic, "CTRB_RESTART");
+    hi.version = 2;
+    hi.header_bytes = (uint32_t)hb;
+    hi.real_bytes = 8;
+    hi.cplx_bytes = 16;
+    hi.Nbase = hi.N = hi.NX = hi.NZ = h->N;
+    hi.NK = hi.NK_full = h->NK;
+    hi.NX_full = hi.NZ_full = h->M;
+    hi.it = h->sc.it;
+    hi.ifn = h->sc.ifn;
+    std::memcpy(head.data(), &hi, sizeof hi);
+    const double sc[kScalarCount] = {h->sc.Re, h->sc.K0, h->sc.visc, h->sc.t, h->sc.dt,
+                                     h->sc.cn, h->sc.cnm1, h->sc.cflnum, h->sc.cflm};
+    std::memcpy(head.data() + 72, sc, sizeof sc);
+    const uint64_t om_count = (uint64_t)h->H * h->N;
+    const uint64_t ur_count = (uint64_t)h->M * h->M * 3;
+    const uint64_t counts[4] = {0, om_count, om_count, ur_count};  // uc, om2, fnm1, ur
+    std::memcpy(head.data() + align8(72 + sizeof sc), counts, sizeof counts);
+
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return fail(h, DNS_E_IO, "cannot open %s for writing: %s", path, std::strerror(errno));
+    bool ok = write_all(fp, head.data(), hb);
+    std::vector<double> plane((size_t)h->M * h->M);
+    const int comps[3] = {DNS_FIELD_U, DNS_FIELD_W, DNS_FIELD_OMEGA};
+    for (int c = 0; ok && c < 3; ++c) {
+        dns_status st = dns_fetch_field(h, comps[c], plane.data());
+        if (st != DNS_OK) {
+            std::fclose(fp);
+            return st;
+        }
+        ok = write_all(fp, plane.data(), sizeof(double) * plane.size());
+    }
+    std::vector<double> spec(2 * (size_t)om_count);
+    for (int which = 0; ok && which < 2; ++which) {
+        dns_status st = which == 0 ? dns_get_om2(h, spec.data()) : dns_get_fnm1(h, spec.data());
+        if (st != DNS_OK) {
+            std::fclose(fp);
+            return st;
+        }
+        ok = write_all(fp, spec.data(), sizeof(double) * spec.size());
+    }
+    if (std::fclose(fp) != 0) ok = false;
+    if (!ok) return fail(h, DNS_E_IO, "write to %s failed", path);
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_load_restart(dns_handle *h, const char *path)
+{
+    if (!h || !path) return DNS_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return fail(h, DNS_E_IO, "cannot open %s: %s", path, std::strerror(errno));
+    struct Closer {
+        FILE *f;
+        ~Closer() { std::fclose(f); }
+    } closer{fp};
+    HeadInts hi{};
+    if (!read_all(fp, &hi, sizeof hi)) return fail(h, DNS_E_FORMAT, "%s: truncated header", path);
+    if (std::strncmp(hi.magic, "CTRB_RESTART", 12) != 0) return fail(h, DNS_E_FORMAT, "%s: bad magic", path);
+    if (hi.version < 1 || hi.version > 2) return fail(h, DNS_E_FORMAT, "%s: unsupported version %u", path, hi.version);
+    const size_t rb = hi.real_bytes;
+    if (!((rb == 4 && hi.cplx_bytes == 8) || (rb == 8 && hi.cplx_bytes == 16)))
+        return fail(h, DNS_E_FORMAT, "%s: unsupported real/cplx sizes %u/%u", path, hi.real_bytes, hi.cplx_bytes);
+    const size_t want_hb = hi.version == 2 ? v2_bytes(rb) : v1_bytes(rb);
+    if (hi.header_bytes != want_hb)
+        return fail(h, DNS_E_FORMAT, "%s: header size %u, expected %zu", path, hi.header_bytes, want_hb);
+    if (hi.Nbase != h->N || hi.N != h->N || hi.NX != h->N || hi.NZ != h->N || hi.NK != h->NK ||
+        hi.NX_full != h->M || hi.NZ_full != h->M || hi.NK_full != h->NK)
+        return fail(h, DNS_E_FORMAT, "%s: restart dimensions do not match this run (N=%d)", path, h->N);
+    std::vector<unsigned char> rest(want_hb - sizeof hi);
+    if (!read_all(fp, rest.data(), rest.size())) return fail(h, DNS_E_FORMAT, "%s: truncated header", path);
+    double sc[kScalarCount];
+    for (size_t i = 0; i < kScalarCount; ++i) {
+        if (rb == 8) {
+            std::memcpy(&sc[i], rest.data() + i * 8, 8);
+        } else {
+            float f;
+            std::memcpy(&f, rest.data() + i * 4, 4);
+            sc[i] = f;
+        }
+    }
+    uint64_t cnt[4] = {0, 0, 0, 0};
+    const size_t coff = align8(72 + kScalarCount * rb) - 72;
+    std::memcpy(cnt, rest.data() + coff, (hi.version == 2 ? 4 : 3) * sizeof(uint64_t));
+    const uint64_t uc_count = (uint64_t)h->NK * h->M * 3, ur_count = (uint64_t)h->M * h->M * 3;
+    const uint64_t om_count = (uint64_t)h->H * h->N;
+    const bool has_uc = cnt[0] != 0;
+    if ((hi.version == 1 && cnt[0] != uc_count) || (hi.version == 2 && has_uc && cnt[0] != uc_count) ||
+        (hi.version == 2 && cnt[3] != ur_count) || cnt[1] != om_count || cnt[2] != om_count)
+        return fail(h, DNS_E_FORMAT, "%s: restart array sizes do not match this run", path);
+    // UC / UR payloads: the velocity is rebuilt from om2, so they are skipped
+    const uint64_t skip = (has_uc ? cnt[0] * 2 * rb : 0) + (hi.version == 2 ? cnt[3] * rb : 0);
+    if (std::fseek(fp, (long)skip, SEEK_CUR) != 0) return fail(h, DNS_E_FORMAT, "%s: truncated payload", path);
+    std::vector<double> om(2 * (size_t)om_count), fnv(2 * (size_t)om_count);
+    for (int which = 0; which < 2; ++which) {
+        std::vector<double> &dst = which == 0 ? om : fnv;
+        if (rb == 8) {
+            if (!read_all(fp, dst.data(), sizeof(double) * dst.size()))
+                return fail(h, DNS_E_FORMAT, "%s: truncated payload", path);
+        } else {
+            std::vector<float> tmp(dst.size());
+            if (!read_all(fp, tmp.data(), sizeof(float) * tmp.size()))
+                return fail(h, DNS_E_FORMAT, "%s: truncated payload", path);
+            for (size_t i = 0; i < tmp.size(); ++i) dst[i] = tmp[i];
+        }
+    }
+    h->sc.Re = sc[0];
+    h->sc.K0 = sc[1];
+    h->sc.visc = sc[2];
+    h->sc.t = sc[3];
+    h->sc.dt = sc[4];
+    h->sc.cn = sc[5];
+    h->sc.cnm1 = sc[6];
+    h->sc.cflnum = sc[7];
+    h->sc.cflm = sc[8];
+    h->sc.it = hi.it;
+    h->sc.ifn = hi.ifn;
+    dns_status st = dns_set_fnm1(h, fnv.data());
+    if (st != DNS_OK) return st;
+    return dns_set_om2(h, om.data());
+}
+
+// =====================================================================================
+// measurement
+// =====================================================================================
+extern "C" dns_status dns_set_kernel_timing(dns_handle *h, int enabled)
+{
+    if (!h) return DNS_E_ARG;
+    drain_timing(h);
+    h->timing = enabled != 0;
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_reset_kernel_timing(dns_handle *h)
+{
+    if (!h) return DNS_E_ARG;
+    drain_timing(h);
+    for (int i = 0; i < KT_COUNT; ++i) {
+        h->launches[i] = 0;
+        h->total_ms[i] = 0.0;
+    }
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_kernel_timing(dns_handle *h, const char **names, int64_t *launches, double *total_ms,
+                                        double *bytes_per_launch, int *n)
+{
+    if (!h || !n) return DNS_E_ARG;
+    drain_timing(h);
+    const double N2 = double(h->N) * double(h->N);
+    // algorithmic HBM bytes per launch (FP64 complex = 16 B):
+    //   k_row     reads 4 fields M x H, writes 1: 5 * 1.5 N * 0.5 N * 16 = 60 N^2
+    //   k_col     reads G (12 N^2) + om, fn (16 N^2), writes om, fn (16 N^2) + V (48 N^2) = 92 N^2
+    //   k_row_cfl reads 2 fields: 24 N^2
+    const double bytes[KT_COUNT] = {60.0 * N2, 92.0 * N2, 24.0 * N2};
+    const int cnt = std::min(*n, (int)KT_COUNT);
+    for (int i = 0; i < cnt; ++i) {
+        if (names) names[i] = kKernelNames[i];
+        if (launches) launches[i] = h->launches[i];
+        if (total_ms) total_ms[i] = h->total_ms[i];
+        if (bytes_per_launch) bytes_per_launch[i] = bytes[i];
+    }
+    *n = KT_COUNT;
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_synchronize(dns_handle *h)
+{
+    if (!h) return DNS_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DNS_OK;
+}
+
+extern "C" size_t dns_device_bytes(const dns_handle *h) { return h ? h->bytes : 0; }

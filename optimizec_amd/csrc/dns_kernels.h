@@ -1,0 +1,59 @@
+// dns_kernels.h -- launch interface between the host API (dns_api.cpp) and the
+// gfx950 kernels (dns_kernels.hip).  Internal to the library; not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+
+#include "dns_types.h"
+
+// spectra the column kernel can z-invert (multipliers of omega)
+enum {
+    DNS_F_U = 0,      // -i gamma / K^2      (kx = 0: -i / gamma)
+    DNS_F_W = 1,      //  i alfa / K^2
+    DNS_F_OX = 2,     //  i alfa             (d omega / dx)
+    DNS_F_OZ = 3,     //  i gamma            (d omega / dz)
+    DNS_F_OMEGA = 4,  //  1
+    DNS_F_PSI = 5     //  1 / K^2            (stream function)
+};
+
+struct ColArgs {
+    int do_forward;     // 1: z-forward of G + CN/AB2 update before the inverse
+    int nfields;        // spectra to z-invert into V (field fi at V + fi * vstride)
+    int fields[4];
+    size_t vstride;
+    double divxz;       // 1 / M^2
+    double VT;          // visc * dt / 2
+    double c2, c3;      // dt (2 + cnm1) / 2, -dt cnm1 / 2
+};
+
+enum { LAUNCH_COL = 0, LAUNCH_ROW = 1, LAUNCH_ROW_CFL = 2, LAUNCH_ROW_PHYS = 3 };
+
+struct LaunchReq {
+    int what;
+    hipStream_t stream;
+    // column kernel
+    dnsfft::cd *om, *fn;
+    const dnsfft::cd *G;
+    dnsfft::cd *V;
+    const dnsfft::cd *tw;
+    ColArgs col;
+    // row kernels
+    size_t vstride;
+    dnsfft::cd *Gw;
+    double *cflrow;
+    double DX, DZ;
+    const dnsfft::cd *Va, *Vb;
+    double *out0, *out1;
+    int out_mode;
+};
+
+int dns_kernels_supported(int H);
+int dns_kernels_col_tile(int H);
+hipError_t dns_kernels_launch(int H, const LaunchReq &r);
+hipError_t dns_launch_energy(const dnsfft::cd *om, int N, int H, double *partE, double *partZ, hipStream_t s);
+hipError_t dns_launch_minmax(const double *f, size_t n, double *out, int blocks, hipStream_t s);
+hipError_t dns_launch_pixels(const double *f, size_t n, double mn, double rng, int is_const,
+                             unsigned long long *out, int blocks, hipStream_t s);
+hipError_t dns_launch_transpose(const dnsfft::cd *in, dnsfft::cd *out, int rows, int cols, hipStream_t s);

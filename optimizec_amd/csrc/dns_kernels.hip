@@ -1,0 +1,621 @@
+// dns_kernels.hip -- the time step of the 2D pseudo-spectral vorticity solver as
+// hand-written gfx950 kernels (FP64).
+//
+// Grid: base N, H = N/2 retained kx (and kz = -H..H-1), M = 3N/2 = 3H physical points.
+// Every M-point transform of the 3/2-padded grid is split as M = 3 x H:
+//   inverse  x[z2 + 3 z1] = sum_k1 W_H^{-z1 k1} W_M^{-z2 k1} (X[k1] + X[k1+2H] W_3^{-2 z2})
+//   forward  X[k1 + H k2] = sum_z2 W_M^{z2 k1} W_3^{z2 k2} F_z2[k1],
+//            F_z2 = H-point FFT of x[z2 + 3 z1]
+// where the zero-padded band k in [H, 2H) never enters (pruned) and only the retained
+// outputs k2 in {0, 2} are formed.  Each M-point transform is therefore three
+// independent H-point FFTs (LDS-resident, fft_lds.hpp) plus a radix-3 combine in
+// registers.
+//
+// Device layouts (all FP64 complex = 16 B):
+//   om, fn : [kx][iz]   column-contiguous (H columns of N)          -- state
+//   V      : [f][z][kx] 4 fields (u, w, d_x omega, d_z omega) M x H -- z-inverse outputs
+//   G      : [z][kx]    M x H                                     -- x-forward of u.grad(omega)
+//   tw     : W_M^t = e^{-2 pi i t / M}, t in [0, M)
+//
+// Per step two kernels run (reference step order mem_cuda.cu:1851-1857):
+//   k_row  (z-row per workgroup)  : V rows -> x-inverse (u + i w, wx + i wz) -> product
+//                                   P = u wx + w wz (+ CFL max) -> x-forward of P -> G row
+//                                   [replaces STEP2B: step2b_cuda.cu:137-276 + cuFFT R2C]
+//   k_col  (C columns per group)  : G columns -> z-forward -> FN = -P^/M^2 -> CN/AB2 update
+//                                   of om, fn -> 4 spectra of the new omega -> z-inverse -> V
+//                                   [replaces STEP3 + STEP2A: step3_cuda.cu:35-173,
+//                                    step2a_cuda.cu:59-261 + cuFFT C2R]
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "fft_lds.hpp"
+#include "dns_kernels.h"
+
+using dnsfft::cd;
+using namespace dnsfft;
+
+namespace {
+
+// wavenumber of compact row iz (pao_km3_cuda.cu:87-92): iz for iz < N/2, iz - N otherwise
+__device__ __forceinline__ double gamma_of(int iz, int N) { return (iz < (N >> 1)) ? double(iz) : double(iz - N); }
+
+// Spectrum of field f at (kx, kz) from omega: every field is s * omega or i s * omega
+// with s real (step3_cuda.cu:102-138 for u, w; om2phys_cuda.cu:216-251 for psi).
+// inv_k2 = 1 / (K^2 + 1e-30) and inv_g = 1 / gamma (0 where the reference sets 0) are
+// computed once per mode by the caller, so the per-field work is multiplications only.
+__device__ __forceinline__ cd field_spec(int f, cd om, int ix, int iz, double ax, double gz, double inv_k2,
+                                         double inv_g)
+{
+    double s;
+    bool times_i = true;
+    switch (f) {
+        case DNS_F_U:  // -i gamma / K^2; kx = 0: -i / gamma
+            s = (ix >= 1) ? -gz * inv_k2 : -inv_g;
+            break;
+        case DNS_F_W:  // i alfa / K^2; kx = 0: 0
+            s = (ix >= 1) ? ax * inv_k2 : 0.0;
+            break;
+        case DNS_F_OX:
+            s = ax;
+            break;
+        case DNS_F_OZ:
+            s = gz;
+            break;
+        case DNS_F_PSI:
+            s = (ix == 0 && iz == 0) ? 0.0 : inv_k2;
+            times_i = false;
+            break;
+        default:  // DNS_F_OMEGA
+            s = 1.0;
+            times_i = false;
+            break;
+    }
+    return times_i ? mk(-s * om.y, s * om.x) : mk(s * om.x, s * om.y);
+}
+
+__device__ __forceinline__ double inv_k2_of(double ax, double gz) { return 1.0 / (ax * ax + gz * gz + 1.0e-30); }
+__device__ __forceinline__ double inv_g_of(int iz, double gz) { return (iz >= 1 && gz != 0.0) ? 1.0 / gz : 0.0; }
+
+}  // namespace
+
+// =====================================================================================
+// Column kernel: z-direction work on C adjacent kx columns.
+// =====================================================================================
+template <int H, int C, int T>
+__global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
+                                           const cd *__restrict__ G, cd *__restrict__ V,
+                                           const cd *__restrict__ tw, ColArgs a)
+{
+    constexpr int N = 2 * H;
+    constexpr int M = 3 * H;
+    constexpr int PJ = C * H / T;
+    static_assert((C * H) % T == 0, "tile must divide evenly over the threads");
+    __shared__ cd lds[C * H];
+
+    const int tid = threadIdx.x;
+    const int kx0 = blockIdx.x * C;
+
+    // registers: the (lo, hi) pair of each owned (column c, k1): iz = k1 and iz = k1 + H
+    cd xlo[PJ], xhi[PJ];
+
+    if (a.do_forward) {
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) xlo[j] = xhi[j] = mk(0.0, 0.0);
+#pragma unroll
+        for (int z2 = 0; z2 < 3; ++z2) {
+            // gather rows z = 3 z1 + z2 of the C columns, transposed into lines
+            for (int e = tid; e < C * H; e += T) {
+                const int c = e % C, z1 = e / C;
+                lds[c * H + z1] = G[(size_t)(3 * z1 + z2) * H + kx0 + c];
+            }
+            __syncthreads();
+            fft_lines<H, H, C, T, false>(lds, tw, 3);
+            const cd w3 = tw[(2 * z2 * H) % M];  // W_3^{2 z2}
+#pragma unroll
+            for (int j = 0; j < PJ; ++j) {
+                const int p = tid + j * T;
+                const int c = p / H, k1 = p - (p / H) * H;
+                const cd F = mul(lds[c * H + k1], tw[opaque(z2 * k1)]);  // W_M^{z2 k1} F_z2
+                xlo[j] = add(xlo[j], F);
+                xhi[j] = add(xhi[j], mul(F, w3));
+            }
+            __syncthreads();
+        }
+        // CN/AB2 update (step3_cuda.cu:75-100): FN = -P^/M^2, row kz = -N/2 forced to 0
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const int p = tid + j * T;
+            const int c = p / H, k1 = p - (p / H) * H;
+            const int kx = kx0 + c;
+            const double ax = double(kx);
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int iz = k1 + half * H;
+                const double gz = gamma_of(iz, N);
+                const double K2 = ax * ax + gz * gz;
+                const cd X = half ? xhi[j] : xlo[j];
+                cd FN = mk(-X.x * a.divxz, -X.y * a.divxz);
+                if (iz == H) FN = mk(0.0, 0.0);
+                const size_t io = (size_t)kx * N + iz;
+                const cd o = om[io], f = fn[io];
+                const double ARG = a.VT * K2;
+                const double DEN = 1.0 + ARG;
+                const double c1 = 1.0 - ARG;
+                const cd num = mk(c1 * o.x + a.c2 * FN.x + a.c3 * f.x, c1 * o.y + a.c2 * FN.y + a.c3 * f.y);
+                const cd on = mk(num.x / DEN, num.y / DEN);
+                om[io] = on;
+                fn[io] = FN;
+                if (half)
+                    xhi[j] = on;
+                else
+                    xlo[j] = on;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const int p = tid + j * T;
+            const int c = p / H, k1 = p - (p / H) * H;
+            const size_t base = (size_t)(kx0 + c) * N;
+            xlo[j] = om[base + k1];
+            xhi[j] = om[base + k1 + H];
+        }
+    }
+
+    // per-mode reciprocals shared by all fields
+    double ik_lo[PJ], ik_hi[PJ];
+#pragma unroll
+    for (int j = 0; j < PJ; ++j) {
+        const int p = tid + j * T;
+        const int c = p / H, k1 = p - (p / H) * H;
+        const double ax = double(kx0 + c);
+        ik_lo[j] = inv_k2_of(ax, gamma_of(k1, N));
+        ik_hi[j] = inv_k2_of(ax, gamma_of(k1 + H, N));
+    }
+
+    // z-inverse of the requested spectra of the (new) omega
+    for (int fi = 0; fi < a.nfields; ++fi) {
+        const int f = a.fields[fi];
+        cd *out = V + (size_t)fi * a.vstride;
+#pragma unroll
+        for (int z2 = 0; z2 < 3; ++z2) {
+            const cd w3 = tw[(2 * z2 * H) % M];
+#pragma unroll
+            for (int j = 0; j < PJ; ++j) {
+                const int p = tid + j * T;
+                const int c = p / H, k1 = p - (p / H) * H;
+                const int kx = kx0 + c;
+                const double ax = double(kx);
+                const double glo = gamma_of(k1, N), ghi = gamma_of(k1 + H, N);
+                // 1/gamma is only needed on the kx = 0 column
+                const double iglo = (kx == 0) ? inv_g_of(k1, glo) : 0.0;
+                const double ighi = (kx == 0) ? inv_g_of(k1 + H, ghi) : 0.0;
+                const cd slo = field_spec(f, xlo[j], kx, k1, ax, glo, ik_lo[j], iglo);
+                const cd shi = field_spec(f, xhi[j], kx, k1 + H, ax, ghi, ik_hi[j], ighi);
+                // W_M^{-z2 k1} (S[k1] + S[k1 + 2H] W_3^{-2 z2})
+                lds[c * H + k1] = mulc(add(slo, mulc(shi, w3)), tw[opaque(z2 * k1)]);
+            }
+            __syncthreads();
+            fft_lines<H, H, C, T, true>(lds, tw, 3);
+            for (int e = tid; e < C * H; e += T) {
+                const int c = e % C, z1 = e / C;
+                out[(size_t)(3 * z1 + z2) * H + kx0 + c] = lds[c * H + z1];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// =====================================================================================
+// Row kernel: x-direction work on one z-row of the padded grid.
+// MODE 0: full nonlinear row (inverse of u+iw and wx+iwz, product, CFL max, forward).
+// MODE 1: velocity only (inverse of u+iw, CFL max) -- NEXTDT (dns_cuda.cu:485-547).
+// =====================================================================================
+template <int H, int T, int MODE>
+__global__ __launch_bounds__(T) void k_row(const cd *__restrict__ V, size_t vstride, cd *__restrict__ G,
+                                           const cd *__restrict__ tw, double *__restrict__ cflrow,
+                                           double DX, double DZ)
+{
+    constexpr int M = 3 * H;
+    constexpr int NL = (MODE == 0) ? 2 : 1;
+    constexpr int PJ = (H + T - 1) / T;
+    __shared__ cd lds[2 * H];
+    __shared__ double red[T / 64];
+    cd *A = lds;
+    cd *B = lds + H;
+    const int tid = threadIdx.x;
+    const int z = blockIdx.x;
+    const cd *u = V + (size_t)z * H;
+    const cd *w = u + vstride;
+    const cd *ox = u + 2 * vstride;
+    const cd *oz = u + 3 * vstride;
+
+    double P[3][PJ];
+    double cmax = 0.0;
+
+#pragma unroll
+    for (int x2 = 0; x2 < 3; ++x2) {
+        const cd w3 = tw[(2 * x2 * H) % M];
+        for (int k1 = tid; k1 < H; k1 += T) {
+            // Z[k] = a[k] + i b[k] on the full M-point spectrum (Hermitian extension of two
+            // real fields); bin 0 keeps only real parts (C2R convention), k = 2H is not
+            // retained (kx = -N/2)
+            const cd tk = tw[opaque(x2 * k1)];
+            {
+                const cd av = u[k1], bv = w[k1];
+                cd zlo = (k1 == 0) ? mk(av.x, bv.x) : mk(av.x - bv.y, av.y + bv.x);
+                cd zhi = mk(0.0, 0.0);
+                if (k1 > 0) {
+                    const cd a2 = u[H - k1], b2 = w[H - k1];
+                    zhi = mk(a2.x + b2.y, b2.x - a2.y);
+                }
+                A[k1] = mulc(add(zlo, mulc(zhi, w3)), tk);
+            }
+            if (MODE == 0) {
+                const cd av = ox[k1], bv = oz[k1];
+                cd zlo = (k1 == 0) ? mk(av.x, bv.x) : mk(av.x - bv.y, av.y + bv.x);
+                cd zhi = mk(0.0, 0.0);
+                if (k1 > 0) {
+                    const cd a2 = ox[H - k1], b2 = oz[H - k1];
+                    zhi = mk(a2.x + b2.y, b2.x - a2.y);
+                }
+                B[k1] = mulc(add(zlo, mulc(zhi, w3)), tk);
+            }
+        }
+        __syncthreads();
+        fft_lines<H, H, NL, T, true>(lds, tw, 3);
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const int x1 = tid + j * T;
+            if (x1 < H) {
+                const cd vel = A[x1];  // (u, w) at x = x2 + 3 x1
+                const double c = fabs(vel.x) / DX + fabs(vel.y) / DZ;
+                cmax = c > cmax ? c : cmax;
+                if (MODE == 0) {
+                    const cd g = B[x1];
+                    P[x2][j] = vel.x * g.x + vel.y * g.y;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    if (MODE == 0) {
+        // forward of the real product: line A = P0 + i P1, line B = P2
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const int x1 = tid + j * T;
+            if (x1 < H) {
+                A[x1] = mk(P[0][j], P[1][j]);
+                B[x1] = mk(P[2][j], 0.0);
+            }
+        }
+        __syncthreads();
+        fft_lines<H, H, 2, T, false>(lds, tw, 3);
+        cd *g = G + (size_t)z * H;
+        for (int k = tid; k < H; k += T) {
+            const cd za = A[k];
+            const cd zm = conj(A[k == 0 ? 0 : H - k]);
+            const cd s = add(za, zm), d = sub(za, zm);
+            const cd F0 = mk(0.5 * s.x, 0.5 * s.y);
+            const cd F1 = mk(0.5 * d.y, -0.5 * d.x);  // d / (2 i)
+            const cd F2 = B[k];
+            g[k] = add(F0, add(mul(F1, tw[opaque(k)]), mul(F2, tw[opaque(2 * k)])));
+        }
+    }
+
+    if (cflrow) {
+        // wavefront max, then across the workgroup's waves
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(cmax, off, 64);
+            cmax = o > cmax ? o : cmax;
+        }
+        if ((tid & 63) == 0) red[tid >> 6] = cmax;
+        __syncthreads();
+        if (tid == 0) {
+            double m = red[0];
+            for (int i = 1; i < T / 64; ++i) m = red[i] > m ? red[i] : m;
+            cflrow[z] = m;
+        }
+    }
+}
+
+// =====================================================================================
+// Physical plane of one or two z-inverted fields (diagnostics / output dumps).
+// out_mode 0: out0 = Re, out1 = Im (if non-null); out_mode 1: out0 = |a + i b|.
+// =====================================================================================
+template <int H, int T>
+__global__ __launch_bounds__(T) void k_row_phys(const cd *__restrict__ Va, const cd *__restrict__ Vb,
+                                                const cd *__restrict__ tw, double *__restrict__ out0,
+                                                double *__restrict__ out1, int out_mode)
+{
+    constexpr int M = 3 * H;
+    constexpr int PJ = (H + T - 1) / T;
+    __shared__ cd A[H];
+    const int tid = threadIdx.x;
+    const int z = blockIdx.x;
+    const cd *a = Va + (size_t)z * H;
+    const cd *b = Vb ? Vb + (size_t)z * H : nullptr;
+    for (int x2 = 0; x2 < 3; ++x2) {
+        const cd w3 = tw[(2 * x2 * H) % M];
+        for (int k1 = tid; k1 < H; k1 += T) {
+            const cd av = a[k1];
+            const cd bv = b ? b[k1] : mk(0.0, 0.0);
+            cd zlo = (k1 == 0) ? mk(av.x, bv.x) : mk(av.x - bv.y, av.y + bv.x);
+            cd zhi = mk(0.0, 0.0);
+            if (k1 > 0) {
+                const cd a2 = a[H - k1];
+                const cd b2 = b ? b[H - k1] : mk(0.0, 0.0);
+                zhi = mk(a2.x + b2.y, b2.x - a2.y);
+            }
+            A[k1] = mulc(add(zlo, mulc(zhi, w3)), tw[opaque(x2 * k1)]);
+        }
+        __syncthreads();
+        fft_lines<H, H, 1, T, true>(A, tw, 3);
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const int x1 = tid + j * T;
+            if (x1 < H) {
+                const cd v = A[x1];
+                const size_t o = (size_t)z * M + x2 + 3 * x1;
+                if (out_mode == 1) {
+                    out0[o] = sqrt(v.x * v.x + v.y * v.y);
+                } else {
+                    out0[o] = v.x;
+                    if (out1) out1[o] = v.y;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// =====================================================================================
+// Reductions (diagnostics).
+// =====================================================================================
+template <int T>
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < T / 64; ++i) s += red[i];
+    __syncthreads();
+    return s;
+}
+
+// E, Z partial sums per kx column (pao_km3_cuda.cu:366-378 convention)
+__global__ __launch_bounds__(256) void k_energy(const cd *__restrict__ om, int N, double *__restrict__ partE,
+                                                double *__restrict__ partZ)
+{
+    __shared__ double red[4];
+    const int kx = blockIdx.x;
+    const double m = (kx == 0) ? 1.0 : 2.0;
+    const double ax2 = double(kx) * double(kx);
+    double e = 0.0, zz = 0.0;
+    for (int iz = threadIdx.x; iz < N; iz += 256) {
+        const cd o = om[(size_t)kx * N + iz];
+        const double gz = gamma_of(iz, N);
+        const double K2 = ax2 + gz * gz;
+        const double a2 = o.x * o.x + o.y * o.y;
+        zz += m * a2;
+        if (K2 > 0.0) e += m * a2 / K2;
+    }
+    const double se = block_sum<256>(e, red);
+    const double sz = block_sum<256>(zz, red);
+    if (threadIdx.x == 0) {
+        partE[kx] = se;
+        partZ[kx] = sz;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_minmax(const double *__restrict__ f, size_t n, double *__restrict__ out)
+{
+    __shared__ double smin[256], smax[256];
+    double mn = 1.0e308, mx = -1.0e308;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double v = f[i];
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            smin[threadIdx.x] = smin[threadIdx.x + o] < smin[threadIdx.x] ? smin[threadIdx.x + o] : smin[threadIdx.x];
+            smax[threadIdx.x] = smax[threadIdx.x + o] > smax[threadIdx.x] ? smax[threadIdx.x + o] : smax[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = smin[0];
+        out[2 * blockIdx.x + 1] = smax[0];
+    }
+}
+
+// pixel statistics of computeDisplaySigma (mem_cuda.cu:691-740), FP64 quantization
+__global__ __launch_bounds__(256) void k_pixels(const double *__restrict__ f, size_t n, double mn, double rng,
+                                                int is_const, unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long s1[256], s2[256];
+    unsigned long long a = 0, b = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        unsigned long long pix = 128;
+        if (!is_const) {
+            double pf = 1.0 + ((f[i] - mn) / rng) * 254.0;
+            pf = pf < 1.0 ? 1.0 : (pf > 255.0 ? 255.0 : pf);
+            pix = (unsigned long long)(int)pf;
+        }
+        a += pix;
+        b += pix * pix;
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s1[threadIdx.x] += s1[threadIdx.x + o];
+            s2[threadIdx.x] += s2[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = s1[0];
+        out[2 * blockIdx.x + 1] = s2[0];
+    }
+}
+
+// transposes between the host layout [iz][ix] and the device layout [ix][iz]
+__global__ void k_transpose_cd(const cd *__restrict__ in, cd *__restrict__ out, int rows, int cols)
+{
+    __shared__ cd tile[16][17];
+    const int bx = blockIdx.x * 16, by = blockIdx.y * 16;
+    int x = bx + threadIdx.x, y = by + threadIdx.y;
+    if (x < cols && y < rows) tile[threadIdx.y][threadIdx.x] = in[(size_t)y * cols + x];
+    __syncthreads();
+    x = by + threadIdx.x;
+    y = bx + threadIdx.y;
+    if (x < rows && y < cols) out[(size_t)y * rows + x] = tile[threadIdx.x][threadIdx.y];
+}
+
+// =====================================================================================
+// Host-side launchers: one set of template instances per supported H.
+// =====================================================================================
+namespace {
+
+template <int H>
+struct Cfg;
+// H: retained kx; C: columns per column-kernel workgroup; TC / TR: threads per workgroup
+#define DNS_CFG(H_, C_, TC_, TR_)              \
+    template <>                                \
+    struct Cfg<H_> {                           \
+        static constexpr int C = C_;           \
+        static constexpr int TC = TC_;         \
+        static constexpr int TR = TR_;         \
+    };
+DNS_CFG(8, 8, 64, 64)
+DNS_CFG(16, 4, 64, 64)
+DNS_CFG(32, 2, 64, 64)
+DNS_CFG(64, 1, 64, 64)
+DNS_CFG(128, 1, 64, 64)
+DNS_CFG(192, 1, 64, 64)
+DNS_CFG(256, 1, 64, 64)
+DNS_CFG(384, 1, 128, 128)
+DNS_CFG(512, 2, 256, 128)
+DNS_CFG(768, 2, 512, 256)
+DNS_CFG(1024, 4, 1024, 256)
+DNS_CFG(1536, 2, 1024, 512)
+DNS_CFG(2048, 2, 1024, 512)
+#undef DNS_CFG
+
+template <int H>
+hipError_t launch_all(const LaunchReq &r)
+{
+    using Q = Cfg<H>;
+    constexpr int M = 3 * H;
+    switch (r.what) {
+        case LAUNCH_COL:
+            hipLaunchKernelGGL((k_col<H, Q::C, Q::TC>), dim3(H / Q::C), dim3(Q::TC), 0, r.stream, r.om, r.fn,
+                               r.G, r.V, r.tw, r.col);
+            break;
+        case LAUNCH_ROW:
+            hipLaunchKernelGGL((k_row<H, Q::TR, 0>), dim3(M), dim3(Q::TR), 0, r.stream, r.V, r.vstride, r.Gw,
+                               r.tw, r.cflrow, r.DX, r.DZ);
+            break;
+        case LAUNCH_ROW_CFL:
+            hipLaunchKernelGGL((k_row<H, Q::TR, 1>), dim3(M), dim3(Q::TR), 0, r.stream, r.V, r.vstride, r.Gw,
+                               r.tw, r.cflrow, r.DX, r.DZ);
+            break;
+        case LAUNCH_ROW_PHYS:
+            hipLaunchKernelGGL((k_row_phys<H, Q::TR>), dim3(M), dim3(Q::TR), 0, r.stream, r.Va, r.Vb, r.tw,
+                               r.out0, r.out1, r.out_mode);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int dns_kernels_supported(int H)
+{
+    switch (H) {
+        case 8: case 16: case 32: case 64: case 128: case 192: case 256: case 384:
+        case 512: case 768: case 1024: case 1536: case 2048:
+            return 1;
+        default:
+            return 0;
+    }
+}
+
+int dns_kernels_col_tile(int H)
+{
+    switch (H) {
+        case 8: return Cfg<8>::C;
+        case 16: return Cfg<16>::C;
+        case 32: return Cfg<32>::C;
+        case 64: return Cfg<64>::C;
+        case 128: return Cfg<128>::C;
+        case 192: return Cfg<192>::C;
+        case 256: return Cfg<256>::C;
+        case 384: return Cfg<384>::C;
+        case 512: return Cfg<512>::C;
+        case 768: return Cfg<768>::C;
+        case 1024: return Cfg<1024>::C;
+        case 1536: return Cfg<1536>::C;
+        case 2048: return Cfg<2048>::C;
+        default: return 0;
+    }
+}
+
+hipError_t dns_kernels_launch(int H, const LaunchReq &r)
+{
+    switch (H) {
+        case 8: return launch_all<8>(r);
+        case 16: return launch_all<16>(r);
+        case 32: return launch_all<32>(r);
+        case 64: return launch_all<64>(r);
+        case 128: return launch_all<128>(r);
+        case 192: return launch_all<192>(r);
+        case 256: return launch_all<256>(r);
+        case 384: return launch_all<384>(r);
+        case 512: return launch_all<512>(r);
+        case 768: return launch_all<768>(r);
+        case 1024: return launch_all<1024>(r);
+        case 1536: return launch_all<1536>(r);
+        case 2048: return launch_all<2048>(r);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t dns_launch_energy(const cd *om, int N, int H, double *partE, double *partZ, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_energy, dim3(H), dim3(256), 0, s, om, N, partE, partZ);
+    return hipGetLastError();
+}
+
+hipError_t dns_launch_minmax(const double *f, size_t n, double *out, int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_minmax, dim3(blocks), dim3(256), 0, s, f, n, out);
+    return hipGetLastError();
+}
+
+hipError_t dns_launch_pixels(const double *f, size_t n, double mn, double rng, int is_const,
+                             unsigned long long *out, int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pixels, dim3(blocks), dim3(256), 0, s, f, n, mn, rng, is_const, out);
+    return hipGetLastError();
+}
+
+hipError_t dns_launch_transpose(const cd *in, cd *out, int rows, int cols, hipStream_t s)
+{
+    dim3 grid((cols + 15) / 16, (rows + 15) / 16);
+    hipLaunchKernelGGL(k_transpose_cd, grid, dim3(16, 16), 0, s, in, out, rows, cols);
+    return hipGetLastError();
+}

@@ -1,0 +1,244 @@
+// fft_lds.hpp -- LDS-resident Stockham FFT engine for gfx950 (wave64).
+//
+// A workgroup transforms NL lines of length L held in LDS (line stride LS complex
+// doubles).  Each radix-R stage: every thread loads the R inputs of its butterflies
+// from LDS into VGPRs (twiddled), the workgroup barriers, the thread runs the radix-R
+// DFT in registers and writes the R outputs to their Stockham (self-sorting)
+// positions, barrier.  The radix sequence is fixed at compile time (8, then 4, 2, 3,
+// 5, 7), so every loop below unrolls and the register arrays are sized exactly.
+//
+// Convention: forward = e^{-2 pi i k n / L}, unnormalized; INV selects e^{+...}.
+// Twiddles come from one table tw[t] = e^{-2 pi i t / M} (t in [0, M)) shared by all
+// line lengths L that divide M: W_L^s = tw[s * (M / L)].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dns_types.h"
+
+namespace dnsfft {
+
+__device__ __forceinline__ cd mk(double a, double b) { return cd{a, b}; }
+__device__ __forceinline__ cd add(cd a, cd b) { return cd{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cd sub(cd a, cd b) { return cd{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cd mul(cd a, cd b)
+{
+    return cd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ cd mulc(cd a, cd b)  // a * conj(b)
+{
+    return cd{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+__device__ __forceinline__ cd scl(cd a, double s) { return cd{a.x * s, a.y * s}; }
+__device__ __forceinline__ cd conj(cd a) { return cd{a.x, -a.y}; }
+// multiply by +i (S>0) or -i (S<0)
+template <int S>
+__device__ __forceinline__ cd muli(cd a)
+{
+    return S > 0 ? cd{-a.y, a.x} : cd{a.y, -a.x};
+}
+
+// Returns x through an empty asm statement: table loads indexed by the result cannot
+// be hoisted out of loops by the compiler (twiddles are loop-invariant in many callers,
+// and keeping all of them live costs more VGPRs than reloading them from L1/L2).
+__device__ __forceinline__ int opaque(int x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+// ---- compile-time radix plan -------------------------------------------------
+__host__ __device__ constexpr int pick_radix(int m)
+{
+    return (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3
+         : (m % 5 == 0) ? 5 : (m % 7 == 0) ? 7 : m;
+}
+__host__ __device__ constexpr int stage_radix(int L, int s)
+{
+    int m = L;
+    for (int i = 0; i < s; ++i) m /= pick_radix(m);
+    return m == 1 ? 1 : pick_radix(m);
+}
+__host__ __device__ constexpr int stage_span(int L, int s)  // product of earlier radices
+{
+    int ns = 1, m = L;
+    for (int i = 0; i < s; ++i) {
+        const int r = pick_radix(m);
+        ns *= r;
+        m /= r;
+    }
+    return ns;
+}
+__host__ __device__ constexpr int num_stages(int L)
+{
+    int n = 0, m = L;
+    while (m > 1) {
+        m /= pick_radix(m);
+        ++n;
+    }
+    return n;
+}
+__host__ __device__ constexpr bool smooth7(int L)
+{
+    int m = L;
+    while (m > 1) {
+        const int r = pick_radix(m);
+        if (r > 8) return false;
+        m /= r;
+    }
+    return true;
+}
+
+// ---- small DFTs in registers ----------------------------------------------------
+// Constants are the exact values cos/sin(2 pi q / R) to double precision.
+template <int R, bool INV>
+__device__ __forceinline__ void dft(cd *v)
+{
+    constexpr int S = INV ? 1 : -1;  // sign of the exponent
+    if constexpr (R == 2) {
+        const cd a = v[0], b = v[1];
+        v[0] = add(a, b);
+        v[1] = sub(a, b);
+    } else if constexpr (R == 4) {
+        const cd s02 = add(v[0], v[2]), d02 = sub(v[0], v[2]);
+        const cd s13 = add(v[1], v[3]), d13 = muli<S>(sub(v[1], v[3]));
+        v[0] = add(s02, s13);
+        v[1] = add(d02, d13);
+        v[2] = sub(s02, s13);
+        v[3] = sub(d02, d13);
+    } else if constexpr (R == 8) {
+        cd e[4] = {v[0], v[2], v[4], v[6]};
+        cd o[4] = {v[1], v[3], v[5], v[7]};
+        dft<4, INV>(e);
+        dft<4, INV>(o);
+        constexpr double h = 0.70710678118654752440084436210484903928;
+        const cd t1 = mul(o[1], cd{h, S * h});
+        const cd t2 = muli<S>(o[2]);
+        const cd t3 = mul(o[3], cd{-h, S * h});
+        v[0] = add(e[0], o[0]);
+        v[4] = sub(e[0], o[0]);
+        v[1] = add(e[1], t1);
+        v[5] = sub(e[1], t1);
+        v[2] = add(e[2], t2);
+        v[6] = sub(e[2], t2);
+        v[3] = add(e[3], t3);
+        v[7] = sub(e[3], t3);
+    } else if constexpr (R == 3) {
+        constexpr double c1 = -0.5;
+        constexpr double s1 = 0.86602540378443864676372317075293618347;
+        const cd sp = add(v[1], v[2]), dm = sub(v[1], v[2]);
+        const cd t = add(v[0], scl(sp, c1));
+        const cd u = muli<S>(scl(dm, s1));
+        v[0] = add(v[0], sp);
+        v[1] = add(t, u);
+        v[2] = sub(t, u);
+    } else if constexpr (R == 5) {
+        constexpr double c1 = 0.30901699437494742410229341718281905886;
+        constexpr double c2 = -0.80901699437494742410229341718281905886;
+        constexpr double s1 = 0.95105651629515357211643933337938214340;
+        constexpr double s2 = 0.58778525229247312916870595463907276860;
+        const cd sp1 = add(v[1], v[4]), dm1 = sub(v[1], v[4]);
+        const cd sp2 = add(v[2], v[3]), dm2 = sub(v[2], v[3]);
+        const cd t1 = add(v[0], add(scl(sp1, c1), scl(sp2, c2)));
+        const cd t2 = add(v[0], add(scl(sp1, c2), scl(sp2, c1)));
+        const cd u1 = muli<S>(add(scl(dm1, s1), scl(dm2, s2)));
+        const cd u2 = muli<S>(sub(scl(dm1, s2), scl(dm2, s1)));
+        v[0] = add(v[0], add(sp1, sp2));
+        v[1] = add(t1, u1);
+        v[4] = sub(t1, u1);
+        v[2] = add(t2, u2);
+        v[3] = sub(t2, u2);
+    } else if constexpr (R == 7) {
+        constexpr double c1 = 0.62348980185873353052500488400423981063;
+        constexpr double c2 = -0.22252093395631440428890256449679475947;
+        constexpr double c3 = -0.90096886790241912623610231950744505116;
+        constexpr double s1 = 0.78183148246802980870844452667405775023;
+        constexpr double s2 = 0.97492791218182360701813168299393121723;
+        constexpr double s3 = 0.43388373911755812047576833284835875461;
+        const cd sp1 = add(v[1], v[6]), dm1 = sub(v[1], v[6]);
+        const cd sp2 = add(v[2], v[5]), dm2 = sub(v[2], v[5]);
+        const cd sp3 = add(v[3], v[4]), dm3 = sub(v[3], v[4]);
+        // q = 1, 2, 3: cos(2 pi r q / 7) and sin(2 pi r q / 7) for r = 1, 2, 3
+        const cd t1 = add(v[0], add(add(scl(sp1, c1), scl(sp2, c2)), scl(sp3, c3)));
+        const cd t2 = add(v[0], add(add(scl(sp1, c2), scl(sp2, c3)), scl(sp3, c1)));
+        const cd t3 = add(v[0], add(add(scl(sp1, c3), scl(sp2, c1)), scl(sp3, c2)));
+        const cd u1 = muli<S>(add(add(scl(dm1, s1), scl(dm2, s2)), scl(dm3, s3)));
+        const cd u2 = muli<S>(sub(sub(scl(dm1, s2), scl(dm2, s3)), scl(dm3, s1)));
+        const cd u3 = muli<S>(add(sub(scl(dm1, s3), scl(dm2, s1)), scl(dm3, s2)));
+        v[0] = add(v[0], add(add(sp1, sp2), sp3));
+        v[1] = add(t1, u1);
+        v[6] = sub(t1, u1);
+        v[2] = add(t2, u2);
+        v[5] = sub(t2, u2);
+        v[3] = add(t3, u3);
+        v[4] = sub(t3, u3);
+    }
+}
+
+// ---- one Stockham stage over NL lines in LDS -----------------------------------
+template <int L, int LS, int NL, int T, int R, int NS, bool INV>
+__device__ __forceinline__ void stage(cd *buf, const cd *__restrict__ tw, int twmul)
+{
+    constexpr int LR = L / R;
+    constexpr int NB = NL * LR;
+    constexpr int PER = (NB + T - 1) / T;
+    constexpr int TWS = L / (NS * R);
+    cd v[PER][R];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int b = tid + q * T;
+        if ((NB % T) == 0 || b < NB) {
+            const int line = b / LR;
+            const int j = b - line * LR;
+            const int k = j % NS;
+            const cd *src = buf + line * LS + j;
+            const int tix = opaque(k * TWS * twmul);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cd a = src[r * LR];
+                if (NS > 1 && r > 0) {
+                    const cd w = tw[tix * r];
+                    a = INV ? mulc(a, w) : mul(a, w);
+                }
+                v[q][r] = a;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int b = tid + q * T;
+        if ((NB % T) == 0 || b < NB) {
+            const int line = b / LR;
+            const int j = b - line * LR;
+            const int k = j % NS;
+            dft<R, INV>(v[q]);
+            cd *dst = buf + line * LS + (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * NS] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, int LS, int NL, int T, bool INV, int S>
+__device__ __forceinline__ void stages_from(cd *buf, const cd *__restrict__ tw, int twmul)
+{
+    if constexpr (S < num_stages(L)) {
+        stage<L, LS, NL, T, stage_radix(L, S), stage_span(L, S), INV>(buf, tw, twmul);
+        stages_from<L, LS, NL, T, INV, S + 1>(buf, tw, twmul);
+    }
+}
+
+// Transform NL lines of length L in place (LDS).  Callers barrier before (data
+// written) -- the first stage reads after the caller's __syncthreads -- and the
+// engine ends with a barrier, so results are visible to every thread on return.
+template <int L, int LS, int NL, int T, bool INV>
+__device__ __forceinline__ void fft_lines(cd *buf, const cd *__restrict__ tw, int twmul)
+{
+    static_assert(smooth7(L), "line length must factor into 2, 3, 5, 7");
+    stages_from<L, LS, NL, T, INV, 0>(buf, tw, twmul);
+}
+
+}  // namespace dnsfft

@@ -1,0 +1,76 @@
+"""CPU-side checks of the product library: it loads, it exports exactly what
+include/dns_hip.h declares, its host-side initial conditions agree with the oracle,
+and device entry points fail loudly (not silently) where no GPU is present."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import optimizec_amd as oa
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    with open(os.path.join(ROOT, "include", "dns_hip.h"), encoding="utf-8") as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(dns_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = oa.load_library()
+    names = _declared_symbols()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    # the Python mirror binds every declared entry point
+    assert sorted(oa.SIGNATURES) == names
+
+
+def test_supported_sizes():
+    for N in (16, 32, 64, 128, 256, 512, 768, 1024, 2048, 3072, 4096):
+        assert oa.supported_n(N), N
+    for N in (10, 30, 100, 8192):
+        assert not oa.supported_n(N), N
+
+
+def test_pao_host_ic_matches_oracle(oracle_mod):
+    om, seed, log, visc = oa.ic_pao_host(512, 1e4, 10.0)
+    o = oracle_mod.Oracle(512, 1e4, 10.0, 0.25)
+    oseed, olog = o.init_pao()
+    assert seed == oseed == 149
+    assert log == olog
+    assert np.array_equal(om, o.om2())
+    assert visc == o.scalars().visc
+
+
+def test_k4exp_host_ic_matches_oracle(oracle_mod):
+    for N, K0, seed in ((64, 4.0, 0), (256, 8.0, 17), (512, 4.0, 3)):
+        om = oa.ic_k4exp_host(N, K0, seed)
+        o = oracle_mod.Oracle(N, 1e4, K0, 0.1)
+        o.init_config(seed)
+        assert np.array_equal(om, o.om2()), (N, K0, seed)
+
+
+def test_create_fails_loudly_without_device():
+    import torch  # noqa: F401  (only to ask whether a device is present)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present; covered by the gpu tests")
+    with pytest.raises(oa.DnsError) as ei:
+        oa.Solver(64, 1e3, 4.0, 0.1)
+    assert ei.value.status == 3  # DNS_E_HIP
+    assert "device" in str(ei.value).lower()
+
+
+def test_bad_arguments_are_rejected():
+    lib = oa.load_library()
+    cfg = oa.Config(30, 1e3, 4.0, 0.1, 0, 0)
+    h = ctypes.c_void_p()
+    assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 1  # DNS_E_ARG: N % 4
+    assert b"multiple of 4" in lib.dns_last_error(None)
+    cfg = oa.Config(8192 * 2, 1e3, 4.0, 0.1, 0, 0)
+    assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 2  # DNS_E_UNSUPPORTED
+    with pytest.raises(oa.DnsError):
+        oa.ic_k4exp_host(30, 4.0, 0)

@@ -1,0 +1,178 @@
+"""GPU parity of the HIP FP64 path against the FP64 CPU oracle, through the C ABI.
+
+Bars (north star): relative L2 vorticity error <= 1e-11 after 100 steps, <= 1e-9 on
+the energy, enstrophy and dt series; single transforms and single steps agree to
+roundoff (<= 1e-13 relative).  At sizes the oracle cannot follow for long (N=4096),
+a few steps are compared directly and the rest is checked through size-independent
+properties (conservation by the nonlinear term, decay of a single shell, determinism).
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import optimizec_amd as oa
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def _pair(oracle_mod, N, Re=1e4, K0=4.0, cfl=0.1, seed=0, kind=oa.IC_K4EXP):
+    s = oa.Solver(N, Re, K0, cfl)
+    s.init_ic(kind, seed)
+    o = oracle_mod.Oracle(N, Re, K0, cfl)
+    if kind == oa.IC_K4EXP:
+        o.init_config(seed)
+    else:
+        o.init_pao()
+    return s, o
+
+
+@pytest.mark.parametrize("N", [16, 64, 256, 768, 1024])
+def test_physical_fields(oracle_mod, N):
+    s, o = _pair(oracle_mod, N, seed=N)
+    assert np.array_equal(s.om2(), o.om2())
+    o.velocity_from_om2()
+    assert relerr(s.field(oa.FIELD_U), o.ur(0)) < 1e-13
+    assert relerr(s.field(oa.FIELD_W), o.ur(1)) < 1e-13
+    o.om2phys()
+    assert relerr(s.field(oa.FIELD_OMEGA), o.ur(2)) < 1e-13
+    kin = np.hypot(o.ur(0), o.ur(1))
+    assert relerr(s.field(oa.FIELD_KINETIC), kin) < 1e-13
+    # psi: -lap(psi) = omega, so its inverse is omega / K^2 padded
+    om = o.om2()
+    kx = np.arange(N // 2)[None, :]
+    kz = np.where(np.arange(N) < N // 2, np.arange(N), np.arange(N) - N)[:, None]
+    k2 = (kx ** 2 + kz ** 2).astype(float)
+    psi = np.where(k2 > 0, om / np.where(k2 > 0, k2, 1.0), 0.0)
+    g = np.zeros((3 * N // 2, 3 * N // 4 + 1), dtype=np.complex128)
+    g[: N // 2, : N // 2] = psi[: N // 2]
+    g[N:, : N // 2] = psi[N // 2:]
+    assert relerr(s.field(oa.FIELD_PSI), oracle_mod.c2r_2d(g)) < 1e-13
+    s.close()
+
+
+@pytest.mark.parametrize("N", [16, 64, 256, 512, 2048])
+def test_start_and_single_step(oracle_mod, N):
+    s, o = _pair(oracle_mod, N, seed=1)
+    s.start()
+    o.start(form=1)
+    ss, so = s.scalars(), o.scalars()
+    assert abs(ss.cflm / so.cflm - 1) < 1e-14
+    assert abs(ss.dt / so.dt - 1) < 1e-14
+    s.step(1)
+    o.one_step(form=1)
+    assert relerr(s.om2(), o.om2()) < 1e-13
+    assert relerr(s.fnm1(), o.fnm1()) < 1e-12
+    s.close()
+
+
+def test_run_512_100_steps(oracle_mod):
+    """Config IC, N=512, Re=1e4, K0=4, CFL=0.1: 100 steps, diagnostics every 20."""
+    N, steps, update = 512, 100, 20
+    s, o = _pair(oracle_mod, N, Re=1e4, K0=4.0, cfl=0.1, seed=0)
+    s.start()
+    o.start(form=1)
+    rs = s.run(steps, update)
+    ro = o.run(steps, update, form=1)
+    assert rs.shape == ro.shape and rs.shape[0] == 1 + steps // update
+    assert np.array_equal(rs[:, 0], ro[:, 0])
+    for col, name in ((1, "t"), (2, "dt"), (3, "cn"), (4, "cflm"), (5, "E"), (6, "Z")):
+        err = np.max(np.abs(rs[:, col] - ro[:, col]) / np.abs(ro[:, col]))
+        assert err < 1e-9, (name, err)
+    err = relerr(s.om2(), o.om2())
+    print(f"N=512 100 steps: rel L2 vorticity error {err:.3e}")
+    assert err < 1e-11
+    s.close()
+
+
+def test_pao_compat_start(oracle_mod):
+    """Reference IC (PAO-k^-3) at N=512: same initial field, same CFLM, 20 steps."""
+    s, o = _pair(oracle_mod, 512, Re=1e4, K0=10.0, cfl=0.25, kind=oa.IC_PAO_KM3)
+    assert np.array_equal(s.om2(), o.om2())
+    s.start()
+    o.start(form=1)
+    assert abs(s.scalars().cflm / o.scalars().cflm - 1) < 1e-14
+    rs = s.run(20, 10)
+    ro = o.run(20, 10, form=1)
+    assert np.max(np.abs(rs[:, 1:] - ro[:, 1:]) / np.abs(ro[:, 1:])) < 1e-10
+    assert relerr(s.om2(), o.om2()) < 1e-12
+    # SIG of the ITERATION line (computeDisplaySigma on OM2PHYS)
+    o.om2phys()
+    assert int(s.display_sigma()) == int(o.display_sigma(2))
+    assert abs(s.display_sigma() - o.display_sigma(2)) < 1e-6
+    s.close()
+
+
+def test_4096_short_parity_and_invariants(oracle_mod):
+    """N=4096 (6144^2 padded), Re=1e5: 3 steps against the oracle, then invariants."""
+    N = 4096
+    s, o = _pair(oracle_mod, N, Re=1e5, K0=4.0, cfl=0.1, seed=0)
+    s.start()
+    o.start(form=1)
+    assert abs(s.scalars().dt / o.scalars().dt - 1) < 1e-13
+    for _ in range(3):
+        s.step(1)
+        o.one_step(form=1)
+    assert relerr(s.om2(), o.om2()) < 1e-13
+    E0, Z0 = s.energy_enstrophy()
+    Eo, Zo = o.energy_enstrophy()
+    assert abs(E0 / Eo - 1) < 1e-13 and abs(Z0 / Zo - 1) < 1e-13
+    s.close()
+
+
+def test_single_shell_decay_gpu():
+    """All modes on |k| = 5: the nonlinear term vanishes and each step multiplies omega
+    by (1 - a)/(1 + a) with a = nu dt K^2 / 2 (closed form, any N)."""
+    N = 256
+    s = oa.Solver(N, 100.0, 4.0, 0.1)
+    om = np.zeros((N, N // 2), dtype=np.complex128)
+    for (kx, kz), v in {(3, 4): 0.3 + 0.1j, (4, -3): -0.2j, (5, 0): 0.15, (0, 5): 0.4 - 0.25j}.items():
+        om[kz % N, kx] = v
+        if kx == 0:
+            om[(-kz) % N, 0] = np.conj(v)
+    s.init_ic(oa.IC_K4EXP, 0)
+    s.set_om2(om)
+    sc = s.scalars()
+    sc.visc, sc.dt, sc.cnm1, sc.cn = 0.01, 0.02, 0.0, 1.0
+    s.set_scalars(sc)
+    s.step(3)
+    a = 0.5 * 0.01 * 0.02 * 25
+    assert relerr(s.om2(), om * ((1 - a) / (1 + a)) ** 3) < 1e-14
+    assert np.max(np.abs(s.fnm1())) < 1e-14
+    s.close()
+
+
+def test_determinism_and_restart_roundtrip():
+    """Two runs are bitwise equal; save at step 5 + load + 5 steps == 10 steps."""
+    N = 256
+    a = oa.Solver(N, 1e4, 4.0, 0.1)
+    a.init_ic(oa.IC_K4EXP, 5)
+    a.start()
+    a.run(10, 5)
+    b = oa.Solver(N, 1e4, 4.0, 0.1)
+    b.init_ic(oa.IC_K4EXP, 5)
+    b.start()
+    b.run(5, 5)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "restart.bin")
+        b.save_restart(path)
+        size = os.path.getsize(path)
+        M = 3 * N // 2
+        assert size == 176 + 8 * 3 * M * M + 2 * 16 * N * (N // 2)
+        c = oa.Solver(N, 1e4, 4.0, 0.1)
+        c.init_ic(oa.IC_K4EXP, 99)
+        c.load_restart(path)
+        sb, sc = b.scalars(), c.scalars()
+        for f in ("it", "t", "dt", "cn", "cnm1", "visc", "cflm", "cflnum"):
+            assert getattr(sb, f) == getattr(sc, f), f
+        c.run(10, 5)
+    assert np.array_equal(a.om2(), c.om2())
+    assert np.array_equal(a.fnm1(), c.fnm1())
+    for s in (a, b, c):
+        s.close()
