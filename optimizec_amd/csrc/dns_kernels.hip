@@ -75,6 +75,14 @@ __device__ __forceinline__ cd field_spec(int f, cd om, int ix, int iz, double ax
     return times_i ? mk(-s * om.y, s * om.x) : mk(s * om.x, s * om.y);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).  Give
+// the workgroups of one XCD consecutive tiles, so that neighbouring column tiles --
+// which share every 128-byte line of the row-major G and V arrays -- meet in one L2.
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+    return (n % 8 == 0) ? (b % 8) * (n / 8) + b / 8 : b;
+}
+
 __device__ __forceinline__ double inv_k2_of(double ax, double gz) { return 1.0 / (ax * ax + gz * gz + 1.0e-30); }
 __device__ __forceinline__ double inv_g_of(int iz, double gz) { return (iz >= 1 && gz != 0.0) ? 1.0 / gz : 0.0; }
 
@@ -91,11 +99,12 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
     constexpr int N = 2 * H;
     constexpr int M = 3 * H;
     constexpr int PJ = C * H / T;
+    constexpr int LS = padded_len(H);
     static_assert((C * H) % T == 0, "tile must divide evenly over the threads");
-    __shared__ cd lds[C * H];
+    __shared__ cd lds[C * LS];
 
     const int tid = threadIdx.x;
-    const int kx0 = blockIdx.x * C;
+    const int kx0 = xcd_tile(blockIdx.x, gridDim.x) * C;
 
     // registers: the (lo, hi) pair of each owned (column c, k1): iz = k1 and iz = k1 + H
     cd xlo[PJ], xhi[PJ];
@@ -108,16 +117,16 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
             // gather rows z = 3 z1 + z2 of the C columns, transposed into lines
             for (int e = tid; e < C * H; e += T) {
                 const int c = e % C, z1 = e / C;
-                lds[c * H + z1] = G[(size_t)(3 * z1 + z2) * H + kx0 + c];
+                lds[c * LS + pidx(z1)] = G[(size_t)(3 * z1 + z2) * H + kx0 + c];
             }
             __syncthreads();
-            fft_lines<H, H, C, T, false>(lds, tw, 3);
+            fft_lines<H, LS, C, T, false>(lds, tw, 3);
             const cd w3 = tw[(2 * z2 * H) % M];  // W_3^{2 z2}
 #pragma unroll
             for (int j = 0; j < PJ; ++j) {
                 const int p = tid + j * T;
                 const int c = p / H, k1 = p - (p / H) * H;
-                const cd F = mul(lds[c * H + k1], tw[opaque(z2 * k1)]);  // W_M^{z2 k1} F_z2
+                const cd F = mul(lds[c * LS + pidx(k1)], tw[opaque(z2 * k1)]);  // W_M^{z2 k1} F_z2
                 xlo[j] = add(xlo[j], F);
                 xhi[j] = add(xhi[j], mul(F, w3));
             }
@@ -195,13 +204,13 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
                 const cd slo = field_spec(f, xlo[j], kx, k1, ax, glo, ik_lo[j], iglo);
                 const cd shi = field_spec(f, xhi[j], kx, k1 + H, ax, ghi, ik_hi[j], ighi);
                 // W_M^{-z2 k1} (S[k1] + S[k1 + 2H] W_3^{-2 z2})
-                lds[c * H + k1] = mulc(add(slo, mulc(shi, w3)), tw[opaque(z2 * k1)]);
+                lds[c * LS + pidx(k1)] = mulc(add(slo, mulc(shi, w3)), tw[opaque(z2 * k1)]);
             }
             __syncthreads();
-            fft_lines<H, H, C, T, true>(lds, tw, 3);
+            fft_lines<H, LS, C, T, true>(lds, tw, 3);
             for (int e = tid; e < C * H; e += T) {
                 const int c = e % C, z1 = e / C;
-                out[(size_t)(3 * z1 + z2) * H + kx0 + c] = lds[c * H + z1];
+                out[(size_t)(3 * z1 + z2) * H + kx0 + c] = lds[c * LS + pidx(z1)];
             }
             __syncthreads();
         }
@@ -212,6 +221,14 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
 // Row kernel: x-direction work on one z-row of the padded grid.
 // MODE 0: full nonlinear row (inverse of u+iw and wx+iwz, product, CFL max, forward).
 // MODE 1: velocity only (inverse of u+iw, CFL max) -- NEXTDT (dns_cuda.cu:485-547).
+//
+// The x-inverse of two real fields a, b packs Z = a + i b on the full M-point
+// spectrum: Z[k] = a[k] + i b[k] (k < H; bin 0 keeps only real parts, the C2R
+// convention) and Z[M - k] = conj(a[k]) + i conj(b[k]).  With M = 3H and
+// x = n2 + 3 n1, each residue n2 is one H-point inverse of
+//   Y[k1] = W_M^{-n2 k1} (Zlo[k1] + Zhi[k1] W_3^{-2 n2}),  Zlo = Z[k1], Zhi = Z[k1 + 2H].
+// Zhi[k1] needs the input at H - k1: the input row is read once, lanes l and l^32 of a
+// wave hold the pair (q, H - q), and one cross-lane exchange gives each its mirror.
 // =====================================================================================
 template <int H, int T, int MODE>
 __global__ __launch_bounds__(T) void k_row(const cd *__restrict__ V, size_t vstride, cd *__restrict__ G,
@@ -220,62 +237,96 @@ __global__ __launch_bounds__(T) void k_row(const cd *__restrict__ V, size_t vstr
 {
     constexpr int M = 3 * H;
     constexpr int NL = (MODE == 0) ? 2 : 1;
-    constexpr int PJ = (H + T - 1) / T;
-    __shared__ cd lds[2 * H];
+    constexpr int LS = padded_len(H);
+    constexpr int PJ = (H / T) > 0 ? (H / T) : 1;  // input pairs per lane
+    constexpr int PO = (H + T - 1) / T;            // outputs per thread
+    static_assert(T % 64 == 0, "whole wavefronts");
+    static_assert(H % T == 0 || H < T, "pair slots must tile the half row");
+    __shared__ cd lds[2 * LS];
     __shared__ double red[T / 64];
     cd *A = lds;
-    cd *B = lds + H;
+    cd *B = lds + LS;
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
     const int z = blockIdx.x;
     const cd *u = V + (size_t)z * H;
     const cd *w = u + vstride;
     const cd *ox = u + 2 * vstride;
     const cd *oz = u + 3 * vstride;
 
-    double P[3][PJ];
+    // ---- read the row once: (Zlo, Zhi) of both field pairs for the owned k
+    cd zloA[PJ], zhiA[PJ], zloB[PJ], zhiB[PJ];
+    int kk[PJ];
+#pragma unroll
+    for (int j = 0; j < PJ; ++j) {
+        const int q = (tid >> 6) * 32 + (lane & 31) + j * (T / 2);
+        const bool valid = q < H / 2;
+        const bool hi = lane >= 32;
+        const int k = !hi ? q : (q == 0 ? H / 2 : H - q);
+        kk[j] = valid ? k : -1;
+        cd a = mk(0.0, 0.0), b = mk(0.0, 0.0), c = mk(0.0, 0.0), d = mk(0.0, 0.0);
+        if (valid) {
+            a = u[k];
+            b = w[k];
+            if (MODE == 0) {
+                c = ox[k];
+                d = oz[k];
+            }
+        }
+        // partner lane (l ^ 32) holds H - k, except in the q = 0 pair (0 and H/2)
+        cd am = mk(__shfl_xor(a.x, 32, 64), __shfl_xor(a.y, 32, 64));
+        cd bm = mk(__shfl_xor(b.x, 32, 64), __shfl_xor(b.y, 32, 64));
+        cd cm = mk(0.0, 0.0), dm = mk(0.0, 0.0);
+        if (MODE == 0) {
+            cm = mk(__shfl_xor(c.x, 32, 64), __shfl_xor(c.y, 32, 64));
+            dm = mk(__shfl_xor(d.x, 32, 64), __shfl_xor(d.y, 32, 64));
+        }
+        if (k == H / 2) {  // its own mirror
+            am = a;
+            bm = b;
+            cm = c;
+            dm = d;
+        }
+        if (k == 0) {
+            zloA[j] = mk(a.x, b.x);
+            zhiA[j] = mk(0.0, 0.0);
+            zloB[j] = mk(c.x, d.x);
+            zhiB[j] = mk(0.0, 0.0);
+        } else {
+            zloA[j] = mk(a.x - b.y, a.y + b.x);
+            zhiA[j] = mk(am.x + bm.y, bm.x - am.y);
+            zloB[j] = mk(c.x - d.y, c.y + d.x);
+            zhiB[j] = mk(cm.x + dm.y, dm.x - cm.y);
+        }
+    }
+
+    double P[3][PO];
     double cmax = 0.0;
 
+#pragma unroll 1
+    for (int n2 = 0; n2 < 3; ++n2) {
+        const cd w3 = tw[opaque((2 * n2 * H) % M)];  // W_3^{2 n2}
 #pragma unroll
-    for (int x2 = 0; x2 < 3; ++x2) {
-        const cd w3 = tw[(2 * x2 * H) % M];
-        for (int k1 = tid; k1 < H; k1 += T) {
-            // Z[k] = a[k] + i b[k] on the full M-point spectrum (Hermitian extension of two
-            // real fields); bin 0 keeps only real parts (C2R convention), k = 2H is not
-            // retained (kx = -N/2)
-            const cd tk = tw[opaque(x2 * k1)];
-            {
-                const cd av = u[k1], bv = w[k1];
-                cd zlo = (k1 == 0) ? mk(av.x, bv.x) : mk(av.x - bv.y, av.y + bv.x);
-                cd zhi = mk(0.0, 0.0);
-                if (k1 > 0) {
-                    const cd a2 = u[H - k1], b2 = w[H - k1];
-                    zhi = mk(a2.x + b2.y, b2.x - a2.y);
-                }
-                A[k1] = mulc(add(zlo, mulc(zhi, w3)), tk);
-            }
-            if (MODE == 0) {
-                const cd av = ox[k1], bv = oz[k1];
-                cd zlo = (k1 == 0) ? mk(av.x, bv.x) : mk(av.x - bv.y, av.y + bv.x);
-                cd zhi = mk(0.0, 0.0);
-                if (k1 > 0) {
-                    const cd a2 = ox[H - k1], b2 = oz[H - k1];
-                    zhi = mk(a2.x + b2.y, b2.x - a2.y);
-                }
-                B[k1] = mulc(add(zlo, mulc(zhi, w3)), tk);
+        for (int j = 0; j < PJ; ++j) {
+            const int k = kk[j];
+            if (k >= 0) {
+                const cd tk = tw[opaque(n2 * k)];
+                A[pidx(k)] = mulc(add(zloA[j], mulc(zhiA[j], w3)), tk);
+                if (MODE == 0) B[pidx(k)] = mulc(add(zloB[j], mulc(zhiB[j], w3)), tk);
             }
         }
         __syncthreads();
-        fft_lines<H, H, NL, T, true>(lds, tw, 3);
+        fft_lines<H, LS, NL, T, true>(lds, tw, 3);
 #pragma unroll
-        for (int j = 0; j < PJ; ++j) {
+        for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
             if (x1 < H) {
-                const cd vel = A[x1];  // (u, w) at x = x2 + 3 x1
-                const double c = fabs(vel.x) / DX + fabs(vel.y) / DZ;
-                cmax = c > cmax ? c : cmax;
+                const cd vel = A[pidx(x1)];  // (u, w) at x = n2 + 3 x1
+                const double cf = fabs(vel.x) / DX + fabs(vel.y) / DZ;
+                cmax = cf > cmax ? cf : cmax;
                 if (MODE == 0) {
-                    const cd g = B[x1];
-                    P[x2][j] = vel.x * g.x + vel.y * g.y;
+                    const cd g = B[pidx(x1)];
+                    P[n2][j] = vel.x * g.x + vel.y * g.y;
                 }
             }
         }
@@ -283,25 +334,26 @@ __global__ __launch_bounds__(T) void k_row(const cd *__restrict__ V, size_t vstr
     }
 
     if (MODE == 0) {
-        // forward of the real product: line A = P0 + i P1, line B = P2
+        // x-forward of the real product: F_n2 = H-point FFT of P[n2 + 3 x1] (lines
+        // A = P0 + i P1, B = P2), then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
 #pragma unroll
-        for (int j = 0; j < PJ; ++j) {
+        for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
             if (x1 < H) {
-                A[x1] = mk(P[0][j], P[1][j]);
-                B[x1] = mk(P[2][j], 0.0);
+                A[pidx(x1)] = mk(P[0][j], P[1][j]);
+                B[pidx(x1)] = mk(P[2][j], 0.0);
             }
         }
         __syncthreads();
-        fft_lines<H, H, 2, T, false>(lds, tw, 3);
+        fft_lines<H, LS, 2, T, false>(lds, tw, 3);
         cd *g = G + (size_t)z * H;
         for (int k = tid; k < H; k += T) {
-            const cd za = A[k];
-            const cd zm = conj(A[k == 0 ? 0 : H - k]);
+            const cd za = A[pidx(k)];
+            const cd zm = conj(A[pidx(k == 0 ? 0 : H - k)]);
             const cd s = add(za, zm), d = sub(za, zm);
             const cd F0 = mk(0.5 * s.x, 0.5 * s.y);
             const cd F1 = mk(0.5 * d.y, -0.5 * d.x);  // d / (2 i)
-            const cd F2 = B[k];
+            const cd F2 = B[pidx(k)];
             g[k] = add(F0, add(mul(F1, tw[opaque(k)]), mul(F2, tw[opaque(2 * k)])));
         }
     }
@@ -312,7 +364,7 @@ __global__ __launch_bounds__(T) void k_row(const cd *__restrict__ V, size_t vstr
             const double o = __shfl_xor(cmax, off, 64);
             cmax = o > cmax ? o : cmax;
         }
-        if ((tid & 63) == 0) red[tid >> 6] = cmax;
+        if (lane == 0) red[tid >> 6] = cmax;
         __syncthreads();
         if (tid == 0) {
             double m = red[0];
@@ -333,7 +385,8 @@ __global__ __launch_bounds__(T) void k_row_phys(const cd *__restrict__ Va, const
 {
     constexpr int M = 3 * H;
     constexpr int PJ = (H + T - 1) / T;
-    __shared__ cd A[H];
+    constexpr int LS = padded_len(H);
+    __shared__ cd A[LS];
     const int tid = threadIdx.x;
     const int z = blockIdx.x;
     const cd *a = Va + (size_t)z * H;
@@ -350,15 +403,15 @@ __global__ __launch_bounds__(T) void k_row_phys(const cd *__restrict__ Va, const
                 const cd b2 = b ? b[H - k1] : mk(0.0, 0.0);
                 zhi = mk(a2.x + b2.y, b2.x - a2.y);
             }
-            A[k1] = mulc(add(zlo, mulc(zhi, w3)), tw[opaque(x2 * k1)]);
+            A[pidx(k1)] = mulc(add(zlo, mulc(zhi, w3)), tw[opaque(x2 * k1)]);
         }
         __syncthreads();
-        fft_lines<H, H, 1, T, true>(A, tw, 3);
+        fft_lines<H, LS, 1, T, true>(A, tw, 3);
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
             const int x1 = tid + j * T;
             if (x1 < H) {
-                const cd v = A[x1];
+                const cd v = A[pidx(x1)];
                 const size_t o = (size_t)z * M + x2 + 3 * x1;
                 if (out_mode == 1) {
                     out0[o] = sqrt(v.x * v.x + v.y * v.y);
@@ -504,13 +557,13 @@ DNS_CFG(32, 2, 64, 64)
 DNS_CFG(64, 1, 64, 64)
 DNS_CFG(128, 1, 64, 64)
 DNS_CFG(192, 1, 64, 64)
-DNS_CFG(256, 1, 64, 64)
+DNS_CFG(256, 1, 64, 128)
 DNS_CFG(384, 1, 128, 128)
-DNS_CFG(512, 2, 256, 128)
+DNS_CFG(512, 2, 256, 256)
 DNS_CFG(768, 2, 512, 256)
-DNS_CFG(1024, 4, 1024, 256)
-DNS_CFG(1536, 2, 1024, 512)
-DNS_CFG(2048, 2, 1024, 512)
+DNS_CFG(1024, 2, 512, 512)
+DNS_CFG(1536, 1, 512, 512)
+DNS_CFG(2048, 1, 512, 512)
 #undef DNS_CFG
 
 template <int H>

@@ -47,6 +47,12 @@ __device__ __forceinline__ int opaque(int x)
     return x;
 }
 
+// LDS position of element i of a line: one 16-byte pad after every 8 elements, so
+// the stride-8 and stride-64 accesses of the radix-8 stages spread over all banks.
+__host__ __device__ constexpr int pidx(int i) { return i + (i >> 3); }
+// padded storage (complex elements) of a line of L elements
+__host__ __device__ constexpr int padded_len(int L) { return L + (L + 7) / 8; }
+
 // ---- compile-time radix plan -------------------------------------------------
 __host__ __device__ constexpr int pick_radix(int m)
 {
@@ -192,11 +198,11 @@ __device__ __forceinline__ void stage(cd *buf, const cd *__restrict__ tw, int tw
             const int line = b / LR;
             const int j = b - line * LR;
             const int k = j % NS;
-            const cd *src = buf + line * LS + j;
+            const cd *src = buf + line * LS;
             const int tix = opaque(k * TWS * twmul);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                cd a = src[r * LR];
+                cd a = src[pidx(j + r * LR)];
                 if (NS > 1 && r > 0) {
                     const cd w = tw[tix * r];
                     a = INV ? mulc(a, w) : mul(a, w);
@@ -214,9 +220,10 @@ __device__ __forceinline__ void stage(cd *buf, const cd *__restrict__ tw, int tw
             const int j = b - line * LR;
             const int k = j % NS;
             dft<R, INV>(v[q]);
-            cd *dst = buf + line * LS + (j / NS) * NS * R + k;
+            cd *dst = buf + line * LS;
+            const int base = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[r * NS] = v[q][r];
+            for (int r = 0; r < R; ++r) dst[pidx(base + r * NS)] = v[q][r];
         }
     }
     __syncthreads();
@@ -231,13 +238,15 @@ __device__ __forceinline__ void stages_from(cd *buf, const cd *__restrict__ tw, 
     }
 }
 
-// Transform NL lines of length L in place (LDS).  Callers barrier before (data
+// Transform NL lines of length L in place (LDS).  Element i of line l lives at
+// buf[l * LS + pidx(i)]; LS >= padded_len(L).  Callers barrier before (data
 // written) -- the first stage reads after the caller's __syncthreads -- and the
 // engine ends with a barrier, so results are visible to every thread on return.
 template <int L, int LS, int NL, int T, bool INV>
 __device__ __forceinline__ void fft_lines(cd *buf, const cd *__restrict__ tw, int twmul)
 {
     static_assert(smooth7(L), "line length must factor into 2, 3, 5, 7");
+    static_assert(LS >= padded_len(L), "line stride must hold the padded line");
     stages_from<L, LS, NL, T, INV, 0>(buf, tw, twmul);
 }
 
