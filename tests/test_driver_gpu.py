@@ -1,0 +1,84 @@
+"""The mem_hip driver on the real solver: its stdout against the reference's own log
+(sim_output_512_1001_gpu.txt, FP32 on an RTX 3090) and a SIGRTMIN snapshot +
+RESTART round trip through the FP64 restart.bin."""
+import json
+import os
+import signal
+import struct
+import subprocess
+import time
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "optimizec_amd", "mem_hip")
+GOLD = os.path.join(ROOT, "tests", "golden", "sim_output_kats.json")
+
+
+def test_driver_matches_reference_log(tmp_path):
+    with open(GOLD, encoding="utf-8") as f:
+        kat = json.load(f)["sim_output_512_1001_gpu"]
+    r = subprocess.run([DRIVER, "512", "1e4", "10", "3", "0.25", "100", "0"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = r.stdout.splitlines()
+    for text in kat["pao_text"].values():
+        assert text in out, text
+    assert f" PAO seed    ={kat['pao_seed']:12d}" in out
+    init = [ln for ln in out if ln.startswith(" [NEXTDT INIT]")][0]
+    cflm = float(init.split("CFLM=")[1].split()[0])
+    assert abs(cflm / kat["nextdt_init"]["CFLM"] - 1) < 3e-7  # FP32 reference
+    it1 = [ln for ln in out if ln.startswith(" ITERATION      1 ")][0]
+    fields = dict(tok.split("=") for tok in it1.split()[2:] if "=" in tok)
+    assert fields["T"] == f"{kat['it1']['T']:.10f}"
+    assert abs(float(fields["CN"]) - kat["it1"]["CN"]) < 1e-7
+    assert abs(float(fields["CFLM"]) / kat["it1"]["CFLM"] - 1) < 3e-7
+    assert int(fields["SIG"]) == kat["it1"]["SIG"]
+
+
+def test_driver_restart_roundtrip(tmp_path):
+    """SIGRTMIN writes output_..._restart/restart.bin (v2, 8-byte reals); RESTART continues
+    to the same state as an uninterrupted run (bitwise: final T line)."""
+    env = dict(os.environ, DNS_IC="k4exp", DNS_SEED="3")
+    args = ["256", "1e4", "4", "0", "0.1", "5", "0"]
+    p = subprocess.Popen([DRIVER, *args], cwd=tmp_path, env=env, stdout=subprocess.PIPE, text=True)
+    t_end = time.time() + 60
+    while time.time() < t_end:
+        ln = p.stdout.readline()
+        if ln.startswith(" ITERATION") and int(ln.split()[1]) >= 20:
+            break
+    p.send_signal(signal.SIGRTMIN)
+    t_end = time.time() + 60
+    while time.time() < t_end:
+        ln = p.stdout.readline()
+        if ln.startswith("[CSV] Wrote") and "meta.csv" in ln:
+            break
+    p.send_signal(signal.SIGINT)
+    p.communicate(timeout=60)
+    assert p.returncode == 128 + signal.SIGINT
+    folder = [d for d in os.listdir(tmp_path) if d.endswith("_restart")][0]
+    path = tmp_path / folder / "restart.bin"
+    with open(path, "rb") as f:
+        head = f.read(176)
+    magic = head[:16].rstrip(b"\0")
+    version, header_bytes, real_bytes, cplx_bytes = struct.unpack("<4I", head[16:32])
+    assert magic == b"CTRB_RESTART" and (version, header_bytes, real_bytes, cplx_bytes) == (2, 176, 8, 16)
+    it_saved = struct.unpack("<10i", head[32:72])[8]
+    N, M = 256, 384
+    assert os.path.getsize(path) == 176 + 8 * 3 * M * M + 2 * 16 * N * (N // 2)
+    stop = it_saved + 13
+    r1 = subprocess.run([DRIVER, "RESTART", folder], cwd=tmp_path, env=dict(env, RESTART_STEPS=str(stop)),
+                        capture_output=True, text=True, timeout=120)
+    assert r1.returncode == 0, r1.stdout + r1.stderr
+    r2 = subprocess.run([DRIVER, "256", "1e4", "4", str(stop), "0.1", "5", "0"], cwd=tmp_path, env=env,
+                        capture_output=True, text=True, timeout=120)
+    assert r2.returncode == 0
+    last1 = [ln for ln in r1.stdout.splitlines() if ln.startswith(" ITERATION")][-1]
+    last2 = [ln for ln in r2.stdout.splitlines() if ln.startswith(" ITERATION")][-1]
+    strip = lambda ln: ln.split(" FPS=")[0]  # noqa: E731
+    assert strip(last1) == strip(last2)
+    final1 = [ln for ln in r1.stdout.splitlines() if ln.startswith(" Final T=")][0]
+    final2 = [ln for ln in r2.stdout.splitlines() if ln.startswith(" Final T=")][0]
+    assert final1 == final2
