@@ -13,7 +13,7 @@
 //
 // Device layouts (all FP64 complex = 16 B):
 //   om, fn : [kx][iz]   column-contiguous (H columns of N)          -- state
-//   V      : [f][z][kx] 4 fields (u, w, d_x omega, d_z omega) M x H -- z-inverse outputs
+//   V      : [f][kx][z] 4 fields (u, w, d_x omega, d_z omega), column-contiguous -- z-inverse outputs
 //   G      : [z][kx]    M x H                                     -- x-forward of u.grad(omega)
 //   tw     : W_M^t = e^{-2 pi i t / M}, t in [0, M)
 //
@@ -89,7 +89,14 @@ __device__ __forceinline__ double inv_g_of(int iz, double gz) { return (iz >= 1 
 }  // namespace
 
 // =====================================================================================
-// Column kernel: z-direction work on C adjacent kx columns.
+// Column kernel: z-direction work on C adjacent kx columns (one workgroup).
+//   forward : the 3C lines z = 3 z1 + z2 of G are transformed together (H-point
+//             forward FFTs), combined with W_M^{z2 k1} W_3^{z2 k2} into the retained
+//             kz, FN = -P^/M^2, CN/AB2 update of om and fn
+//   inverse : for each residue z2, the spectra of the nfields requested fields
+//             (always 4C lines transformed; unused ones are ignored) are pre-twiddled
+//             W_M^{-z2 k1} (S[k1] + S[k1+2H] W_3^{-2 z2}), inverse-transformed and written
+//             to rows z = z2 + 3 z1 of V.
 // =====================================================================================
 template <int H, int C, int T>
 __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
@@ -101,7 +108,7 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
     constexpr int PJ = C * H / T;
     constexpr int LS = padded_len(H);
     static_assert((C * H) % T == 0, "tile must divide evenly over the threads");
-    __shared__ cd lds[C * LS];
+    __shared__ cd lds[3 * C * LS];
 
     const int tid = threadIdx.x;
     const int kx0 = xcd_tile(blockIdx.x, gridDim.x) * C;
@@ -110,27 +117,24 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
     cd xlo[PJ], xhi[PJ];
 
     if (a.do_forward) {
+        for (int e = tid; e < 3 * C * H; e += T) {
+            const int c = e % C, z = e / C;
+            const int z1 = z / 3, z2 = z - 3 * z1;
+            lds[(z2 * C + c) * LS + pidx(z1)] = G[(size_t)z * H + kx0 + c];
+        }
+        __syncthreads();
+        fft_lines<H, LS, 3 * C, T, false>(lds, tw, 3);
+        const cd w3a = tw[2 * H];  // W_3^2
+        const cd w3b = tw[H];      // W_3^4 = W_3
 #pragma unroll
-        for (int j = 0; j < PJ; ++j) xlo[j] = xhi[j] = mk(0.0, 0.0);
-#pragma unroll
-        for (int z2 = 0; z2 < 3; ++z2) {
-            // gather rows z = 3 z1 + z2 of the C columns, transposed into lines
-            for (int e = tid; e < C * H; e += T) {
-                const int c = e % C, z1 = e / C;
-                lds[c * LS + pidx(z1)] = G[(size_t)(3 * z1 + z2) * H + kx0 + c];
-            }
-            __syncthreads();
-            fft_lines<H, LS, C, T, false>(lds, tw, 3);
-            const cd w3 = tw[(2 * z2 * H) % M];  // W_3^{2 z2}
-#pragma unroll
-            for (int j = 0; j < PJ; ++j) {
-                const int p = tid + j * T;
-                const int c = p / H, k1 = p - (p / H) * H;
-                const cd F = mul(lds[c * LS + pidx(k1)], tw[opaque(z2 * k1)]);  // W_M^{z2 k1} F_z2
-                xlo[j] = add(xlo[j], F);
-                xhi[j] = add(xhi[j], mul(F, w3));
-            }
-            __syncthreads();
+        for (int j = 0; j < PJ; ++j) {
+            const int p = tid + j * T;
+            const int c = p / H, k1 = p - (p / H) * H;
+            const cd F0 = lds[(0 * C + c) * LS + pidx(k1)];
+            const cd F1 = mul(lds[(1 * C + c) * LS + pidx(k1)], tw[opaque(k1)]);
+            const cd F2 = mul(lds[(2 * C + c) * LS + pidx(k1)], tw[opaque(2 * k1)]);
+            xlo[j] = add(F0, add(F1, F2));
+            xhi[j] = add(F0, add(mul(F1, w3a), mul(F2, w3b)));
         }
         // CN/AB2 update (step3_cuda.cu:75-100): FN = -P^/M^2, row kz = -N/2 forced to 0
 #pragma unroll
@@ -162,6 +166,7 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
                     xlo[j] = on;
             }
         }
+        __syncthreads();
     } else {
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
@@ -184,36 +189,39 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
         ik_hi[j] = inv_k2_of(ax, gamma_of(k1 + H, N));
     }
 
-    // z-inverse of the requested spectra of the (new) omega
-    for (int fi = 0; fi < a.nfields; ++fi) {
+    // z-inverse of each requested spectrum of the (new) omega: the three residues z2
+    // are transformed together, then each column of V (column-major, z contiguous) is
+    // written whole
+    const int nf = a.nfields;
+#pragma unroll 1
+    for (int fi = 0; fi < nf; ++fi) {
         const int f = a.fields[fi];
-        cd *out = V + (size_t)fi * a.vstride;
 #pragma unroll
-        for (int z2 = 0; z2 < 3; ++z2) {
-            const cd w3 = tw[(2 * z2 * H) % M];
-#pragma unroll
-            for (int j = 0; j < PJ; ++j) {
-                const int p = tid + j * T;
-                const int c = p / H, k1 = p - (p / H) * H;
-                const int kx = kx0 + c;
-                const double ax = double(kx);
-                const double glo = gamma_of(k1, N), ghi = gamma_of(k1 + H, N);
-                // 1/gamma is only needed on the kx = 0 column
-                const double iglo = (kx == 0) ? inv_g_of(k1, glo) : 0.0;
-                const double ighi = (kx == 0) ? inv_g_of(k1 + H, ghi) : 0.0;
-                const cd slo = field_spec(f, xlo[j], kx, k1, ax, glo, ik_lo[j], iglo);
-                const cd shi = field_spec(f, xhi[j], kx, k1 + H, ax, ghi, ik_hi[j], ighi);
-                // W_M^{-z2 k1} (S[k1] + S[k1 + 2H] W_3^{-2 z2})
-                lds[c * LS + pidx(k1)] = mulc(add(slo, mulc(shi, w3)), tw[opaque(z2 * k1)]);
-            }
-            __syncthreads();
-            fft_lines<H, LS, C, T, true>(lds, tw, 3);
-            for (int e = tid; e < C * H; e += T) {
-                const int c = e % C, z1 = e / C;
-                out[(size_t)(3 * z1 + z2) * H + kx0 + c] = lds[c * LS + pidx(z1)];
-            }
-            __syncthreads();
+        for (int j = 0; j < PJ; ++j) {
+            const int p = tid + j * T;
+            const int c = p / H, k1 = p - (p / H) * H;
+            const int kx = kx0 + c;
+            const double ax = double(kx);
+            const double glo = gamma_of(k1, N), ghi = gamma_of(k1 + H, N);
+            // 1/gamma is only needed on the kx = 0 column
+            const double iglo = (kx == 0) ? inv_g_of(k1, glo) : 0.0;
+            const double ighi = (kx == 0) ? inv_g_of(k1 + H, ghi) : 0.0;
+            const cd slo = field_spec(f, xlo[j], kx, k1, ax, glo, ik_lo[j], iglo);
+            const cd shi = field_spec(f, xhi[j], kx, k1 + H, ax, ghi, ik_hi[j], ighi);
+            // residue z2: W_M^{-z2 k1} (S[k1] + S[k1 + 2H] W_3^{-2 z2})
+            lds[(0 * C + c) * LS + pidx(k1)] = add(slo, shi);
+            lds[(1 * C + c) * LS + pidx(k1)] = mulc(add(slo, mulc(shi, tw[2 * H])), tw[opaque(k1)]);
+            lds[(2 * C + c) * LS + pidx(k1)] = mulc(add(slo, mulc(shi, tw[H])), tw[opaque(2 * k1)]);
         }
+        __syncthreads();
+        fft_lines<H, LS, 3 * C, T, true>(lds, tw, 3);
+        cd *out = V + (size_t)fi * a.vstride + (size_t)kx0 * M;
+        for (int e = tid; e < C * M; e += T) {
+            const int c = e / M, zz = e - c * M;
+            const int z1 = zz / 3, z2 = zz - 3 * z1;
+            out[(size_t)c * M + zz] = lds[(z2 * C + c) * LS + pidx(z1)];
+        }
+        __syncthreads();
     }
 }
 
@@ -229,104 +237,86 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
 //   Y[k1] = W_M^{-n2 k1} (Zlo[k1] + Zhi[k1] W_3^{-2 n2}),  Zlo = Z[k1], Zhi = Z[k1 + 2H].
 // Zhi[k1] needs the input at H - k1: the input row is read once, lanes l and l^32 of a
 // wave hold the pair (q, H - q), and one cross-lane exchange gives each its mirror.
+// The three residues of one field pair are transformed together (3 lines).
 // =====================================================================================
 template <int H, int T, int MODE>
 __global__ __launch_bounds__(T) void k_row(const cd *__restrict__ V, size_t vstride, cd *__restrict__ G,
                                            const cd *__restrict__ tw, double *__restrict__ cflrow,
                                            double DX, double DZ)
 {
-    constexpr int M = 3 * H;
-    constexpr int NL = (MODE == 0) ? 2 : 1;
     constexpr int LS = padded_len(H);
     constexpr int PJ = (H / T) > 0 ? (H / T) : 1;  // input pairs per lane
-    constexpr int PO = (H + T - 1) / T;            // outputs per thread
+    constexpr int PO = (H + T - 1) / T;            // outputs per thread and residue
     static_assert(T % 64 == 0, "whole wavefronts");
     static_assert(H % T == 0 || H < T, "pair slots must tile the half row");
-    __shared__ cd lds[2 * LS];
+    __shared__ cd lds[3 * LS];
     __shared__ double red[T / 64];
-    cd *A = lds;
-    cd *B = lds + LS;
+    constexpr int M = 3 * H;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int z = blockIdx.x;
-    const cd *u = V + (size_t)z * H;
-    const cd *w = u + vstride;
-    const cd *ox = u + 2 * vstride;
-    const cd *oz = u + 3 * vstride;
+    // rows in XCD-aware order: neighbouring rows share the 128-byte lines of the
+    // column-major V, so they should meet in one L2
+    const int z = xcd_tile(blockIdx.x, gridDim.x);
+    const cd *row = V + z;  // element (f, k) at f * vstride + k * M
+    // W_3^{2 n2} for n2 = 0, 1, 2 (indices (2 n2 H) mod M)
+    const cd w3_1 = tw[2 * H], w3_2 = tw[H];
 
-    // ---- read the row once: (Zlo, Zhi) of both field pairs for the owned k
-    cd zloA[PJ], zhiA[PJ], zloB[PJ], zhiB[PJ];
-    int kk[PJ];
-#pragma unroll
-    for (int j = 0; j < PJ; ++j) {
-        const int q = (tid >> 6) * 32 + (lane & 31) + j * (T / 2);
-        const bool valid = q < H / 2;
-        const bool hi = lane >= 32;
-        const int k = !hi ? q : (q == 0 ? H / 2 : H - q);
-        kk[j] = valid ? k : -1;
-        cd a = mk(0.0, 0.0), b = mk(0.0, 0.0), c = mk(0.0, 0.0), d = mk(0.0, 0.0);
-        if (valid) {
-            a = u[k];
-            b = w[k];
-            if (MODE == 0) {
-                c = ox[k];
-                d = oz[k];
-            }
-        }
-        // partner lane (l ^ 32) holds H - k, except in the q = 0 pair (0 and H/2)
-        cd am = mk(__shfl_xor(a.x, 32, 64), __shfl_xor(a.y, 32, 64));
-        cd bm = mk(__shfl_xor(b.x, 32, 64), __shfl_xor(b.y, 32, 64));
-        cd cm = mk(0.0, 0.0), dm = mk(0.0, 0.0);
-        if (MODE == 0) {
-            cm = mk(__shfl_xor(c.x, 32, 64), __shfl_xor(c.y, 32, 64));
-            dm = mk(__shfl_xor(d.x, 32, 64), __shfl_xor(d.y, 32, 64));
-        }
-        if (k == H / 2) {  // its own mirror
-            am = a;
-            bm = b;
-            cm = c;
-            dm = d;
-        }
-        if (k == 0) {
-            zloA[j] = mk(a.x, b.x);
-            zhiA[j] = mk(0.0, 0.0);
-            zloB[j] = mk(c.x, d.x);
-            zhiB[j] = mk(0.0, 0.0);
-        } else {
-            zloA[j] = mk(a.x - b.y, a.y + b.x);
-            zhiA[j] = mk(am.x + bm.y, bm.x - am.y);
-            zloB[j] = mk(c.x - d.y, c.y + d.x);
-            zhiB[j] = mk(cm.x + dm.y, dm.x - cm.y);
-        }
-    }
-
-    double P[3][PO];
+    cd vel[3][PO];     // u + i w at x = n2 + 3 x1
+    double P[3][PO];   // u d_x omega + w d_z omega
     double cmax = 0.0;
 
 #pragma unroll 1
-    for (int n2 = 0; n2 < 3; ++n2) {
-        const cd w3 = tw[opaque((2 * n2 * H) % M)];  // W_3^{2 n2}
+    for (int ph = 0; ph < ((MODE == 0) ? 2 : 1); ++ph) {
+        const cd *fa = row + (size_t)(2 * ph) * vstride;
+        const cd *fb = fa + vstride;
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
-            const int k = kk[j];
-            if (k >= 0) {
-                const cd tk = tw[opaque(n2 * k)];
-                A[pidx(k)] = mulc(add(zloA[j], mulc(zhiA[j], w3)), tk);
-                if (MODE == 0) B[pidx(k)] = mulc(add(zloB[j], mulc(zhiB[j], w3)), tk);
+            const int q = (tid >> 6) * 32 + (lane & 31) + j * (T / 2);
+            const bool valid = q < H / 2;
+            const bool hi = lane >= 32;
+            const int k = !hi ? q : (q == 0 ? H / 2 : H - q);
+            cd av = mk(0.0, 0.0), bv = mk(0.0, 0.0);
+            if (valid) {
+                av = fa[(size_t)k * M];
+                bv = fb[(size_t)k * M];
+            }
+            // partner lane (l ^ 32) holds H - k, except in the q = 0 pair (0 and H/2)
+            cd am = mk(__shfl_xor(av.x, 32, 64), __shfl_xor(av.y, 32, 64));
+            cd bm = mk(__shfl_xor(bv.x, 32, 64), __shfl_xor(bv.y, 32, 64));
+            if (k == H / 2) {  // its own mirror
+                am = av;
+                bm = bv;
+            }
+            if (valid) {
+                cd zlo, zhi;
+                if (k == 0) {
+                    zlo = mk(av.x, bv.x);
+                    zhi = mk(0.0, 0.0);
+                } else {
+                    zlo = mk(av.x - bv.y, av.y + bv.x);
+                    zhi = mk(am.x + bm.y, bm.x - am.y);
+                }
+                lds[pidx(k)] = add(zlo, zhi);
+                lds[LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_1)), tw[opaque(k)]);
+                lds[2 * LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_2)), tw[opaque(2 * k)]);
             }
         }
         __syncthreads();
-        fft_lines<H, LS, NL, T, true>(lds, tw, 3);
+        fft_lines<H, LS, 3, T, true>(lds, tw, 3);
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
             if (x1 < H) {
-                const cd vel = A[pidx(x1)];  // (u, w) at x = n2 + 3 x1
-                const double cf = fabs(vel.x) / DX + fabs(vel.y) / DZ;
-                cmax = cf > cmax ? cf : cmax;
-                if (MODE == 0) {
-                    const cd g = B[pidx(x1)];
-                    P[n2][j] = vel.x * g.x + vel.y * g.y;
+#pragma unroll
+                for (int n2 = 0; n2 < 3; ++n2) {
+                    const cd v = lds[n2 * LS + pidx(x1)];
+                    if (ph == 0) {
+                        vel[n2][j] = v;
+                        const double cf = fabs(v.x) / DX + fabs(v.y) / DZ;
+                        cmax = cf > cmax ? cf : cmax;
+                    } else {
+                        P[n2][j] = vel[n2][j].x * v.x + vel[n2][j].y * v.y;
+                    }
                 }
             }
         }
@@ -336,6 +326,8 @@ __global__ __launch_bounds__(T) void k_row(const cd *__restrict__ V, size_t vstr
     if (MODE == 0) {
         // x-forward of the real product: F_n2 = H-point FFT of P[n2 + 3 x1] (lines
         // A = P0 + i P1, B = P2), then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
+        cd *A = lds;
+        cd *B = lds + LS;
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
@@ -389,18 +381,18 @@ __global__ __launch_bounds__(T) void k_row_phys(const cd *__restrict__ Va, const
     __shared__ cd A[LS];
     const int tid = threadIdx.x;
     const int z = blockIdx.x;
-    const cd *a = Va + (size_t)z * H;
-    const cd *b = Vb ? Vb + (size_t)z * H : nullptr;
+    const cd *a = Va + z;  // column-major: element k at k * M
+    const cd *b = Vb ? Vb + z : nullptr;
     for (int x2 = 0; x2 < 3; ++x2) {
         const cd w3 = tw[(2 * x2 * H) % M];
         for (int k1 = tid; k1 < H; k1 += T) {
-            const cd av = a[k1];
-            const cd bv = b ? b[k1] : mk(0.0, 0.0);
+            const cd av = a[(size_t)k1 * M];
+            const cd bv = b ? b[(size_t)k1 * M] : mk(0.0, 0.0);
             cd zlo = (k1 == 0) ? mk(av.x, bv.x) : mk(av.x - bv.y, av.y + bv.x);
             cd zhi = mk(0.0, 0.0);
             if (k1 > 0) {
-                const cd a2 = a[H - k1];
-                const cd b2 = b ? b[H - k1] : mk(0.0, 0.0);
+                const cd a2 = a[(size_t)(H - k1) * M];
+                const cd b2 = b ? b[(size_t)(H - k1) * M] : mk(0.0, 0.0);
                 zhi = mk(a2.x + b2.y, b2.x - a2.y);
             }
             A[pidx(k1)] = mulc(add(zlo, mulc(zhi, w3)), tw[opaque(x2 * k1)]);
@@ -556,13 +548,13 @@ DNS_CFG(16, 4, 64, 64)
 DNS_CFG(32, 2, 64, 64)
 DNS_CFG(64, 1, 64, 64)
 DNS_CFG(128, 1, 64, 64)
-DNS_CFG(192, 1, 64, 64)
-DNS_CFG(256, 1, 64, 128)
-DNS_CFG(384, 1, 128, 128)
-DNS_CFG(512, 2, 256, 256)
-DNS_CFG(768, 2, 512, 256)
-DNS_CFG(1024, 2, 512, 512)
-DNS_CFG(1536, 1, 512, 512)
+DNS_CFG(192, 1, 192, 64)
+DNS_CFG(256, 1, 128, 128)
+DNS_CFG(384, 1, 192, 192)
+DNS_CFG(512, 2, 512, 256)
+DNS_CFG(768, 2, 768, 384)
+DNS_CFG(1024, 2, 1024, 512)
+DNS_CFG(1536, 1, 768, 768)
 DNS_CFG(2048, 1, 512, 512)
 #undef DNS_CFG
 

@@ -199,15 +199,18 @@ __device__ __forceinline__ void stage(cd *buf, const cd *__restrict__ tw, int tw
             const int j = b - line * LR;
             const int k = j % NS;
             const cd *src = buf + line * LS;
-            const int tix = opaque(k * TWS * twmul);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                cd a = src[pidx(j + r * LR)];
-                if (NS > 1 && r > 0) {
-                    const cd w = tw[tix * r];
-                    a = INV ? mulc(a, w) : mul(a, w);
-                }
-                v[q][r] = a;
+            for (int r = 0; r < R; ++r) v[q][r] = src[pidx(j + r * LR)];
+            if (NS > 1) {
+                // w^r for r = 1..R-1 from one table load: w^2 = w w, w^(2m) = (w^m)^2,
+                // w^(2m+1) = w^(2m) w (at most 3 roundings deep for R <= 8)
+                const cd w1 = tw[opaque(k * TWS * twmul)];
+                cd wp[R];
+                wp[1] = w1;
+#pragma unroll
+                for (int r = 2; r < R; ++r) wp[r] = (r % 2 == 0) ? mul(wp[r / 2], wp[r / 2]) : mul(wp[r - 1], w1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[q][r] = INV ? mulc(v[q][r], wp[r]) : mul(v[q][r], wp[r]);
             }
         }
     }
