@@ -68,42 +68,73 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--N", type=int, default=WORKLOAD["N"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
               "multi-GPU path.", file=sys.stderr)
         return 2
 
+    import numpy as np
+
     import optimizec_amd as oa
 
     N = args.N
-    s = oa.Solver(N, WORKLOAD["Re"], WORKLOAD["K0"], WORKLOAD["CFL"])
-    s.init_ic(oa.IC_K4EXP, WORKLOAD["seed"])
-    s.start()
 
-    def advance(n):
+    def make(prec):
+        s = oa.Solver(N, WORKLOAD["Re"], WORKLOAD["K0"], WORKLOAD["CFL"], precision=prec)
+        s.init_ic(oa.IC_K4EXP, WORKLOAD["seed"])
+        s.start()
+        return s
+
+    def advance(s, n):
         for _ in range(n):
             s.step(1)
             it = s.scalars().it
             if it == 1 or it % UPDATE == 0:
                 s.next_dt()
 
-    advance(args.warmup)
-    s.synchronize()
-    s.set_kernel_timing(True)
-    s.reset_kernel_timing()
-    s.synchronize()
-    t0 = time.perf_counter()
-    advance(args.steps)
-    s.synchronize()
-    wall = time.perf_counter() - t0
-    kt = s.kernel_timing()
-    s.set_kernel_timing(False)
+    def timed(s):
+        advance(s, args.warmup)
+        s.synchronize()
+        s.set_kernel_timing(True)
+        s.reset_kernel_timing()
+        s.synchronize()
+        t0 = time.perf_counter()
+        advance(s, args.steps)
+        s.synchronize()
+        w = time.perf_counter() - t0
+        k = s.kernel_timing()
+        s.set_kernel_timing(False)
+        return w, k
+
+    s = make(oa.PREC_FP64)
+    wall, kt = timed(s)
     E, Z = s.energy_enstrophy()
     finite = math.isfinite(E) and math.isfinite(Z)
     sc = s.scalars()
     dev_bytes = s.device_bytes()
     s.close()
+
+    # mixed precision: FP32 transforms, FP64 state / CN-AB2 / CFL-dt / E, Z
+    mixed = None
+    if not args.fp64_only:
+        s32 = make(oa.PREC_FP32)
+        wall32, kt32 = timed(s32)
+        s32.close()
+        a, b = make(oa.PREC_FP64), make(oa.PREC_FP32)
+        advance(a, 100)
+        advance(b, 100)
+        oa64, oa32 = a.om2(), b.om2()
+        err = float(np.linalg.norm(oa32 - oa64) / np.linalg.norm(oa64))
+        a.close()
+        b.close()
+        ms32 = 1e3 * wall32 / args.steps
+        mixed = {"precision": "fp32 transforms + fp64 state", "ms_per_step": round(ms32, 4),
+                 "speedup_vs_fp64": round((1e3 * wall / args.steps) / ms32, 3),
+                 "rel_l2_vorticity_err_vs_fp64_step100": err,
+                 "kernels": {n: round(d["total_ms"] / max(1, d["launches"]), 5) for n, d in kt32.items()
+                             if d["launches"]}}
 
     ms_per_step = 1e3 * wall / args.steps
     per_kernel = {}
@@ -140,6 +171,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": _traffic_from_profile(dom, N)},
+        "mixed_precision": mixed,
         "device_bytes": dev_bytes,
         "final": {"it": sc.it, "t": sc.t, "dt": sc.dt, "E": E, "Z": Z, "finite": finite},
     }

@@ -45,7 +45,10 @@ typedef enum {
 } dns_ic_kind;
 
 typedef enum {
-    DNS_PREC_FP64 = 0
+    DNS_PREC_FP64 = 0,  /* everything in FP64                                              */
+    DNS_PREC_FP32 = 1   /* transforms and their intermediates in FP32; state om2/fnm1, the
+                           CN/AB2 update, CFL/dt, energy/enstrophy and twiddle generation
+                           in FP64 (N >= 128)                                                */
 } dns_precision;
 
 typedef enum {

@@ -18,10 +18,11 @@ import numpy as np
 
 __all__ = ["Solver", "Scalars", "DnsError", "library_path", "load_library", "ic_pao_host",
            "ic_k4exp_host", "supported_n", "FIELD_U", "FIELD_W", "FIELD_OMEGA", "FIELD_PSI",
-           "FIELD_KINETIC", "IC_PAO_KM3", "IC_K4EXP"]
+           "FIELD_KINETIC", "IC_PAO_KM3", "IC_K4EXP", "PREC_FP64", "PREC_FP32"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 IC_PAO_KM3, IC_K4EXP = 0, 1
+PREC_FP64, PREC_FP32 = 0, 1
 FIELD_U, FIELD_W, FIELD_OMEGA, FIELD_PSI, FIELD_KINETIC = 0, 1, 2, 3, 4
 _STATUS = {0: "DNS_OK", 1: "DNS_E_ARG", 2: "DNS_E_UNSUPPORTED", 3: "DNS_E_HIP", 4: "DNS_E_NOMEM",
            5: "DNS_E_STATE", 6: "DNS_E_IO", 7: "DNS_E_FORMAT"}
@@ -136,9 +137,11 @@ def ic_k4exp_host(N: int, K0: float, seed: int) -> np.ndarray:
 class Solver:
     """One device state: the C handle of dns_create (dnsCudaCreate, cuda_dns.h:108)."""
 
-    def __init__(self, N: int, Re: float, K0: float, cflnum: float, device: int = 0):
+    def __init__(self, N: int, Re: float, K0: float, cflnum: float, device: int = 0,
+                 precision: int = PREC_FP64):
+        """precision: PREC_FP64, or PREC_FP32 (FP32 transforms, FP64 state/update/dt)."""
         L = load_library()
-        cfg = Config(N, float(Re), float(K0), float(cflnum), 0, int(device))
+        cfg = Config(N, float(Re), float(K0), float(cflnum), int(precision), int(device))
         h = _P()
         st = L.dns_create(ctypes.byref(cfg), ctypes.byref(h))
         if st:

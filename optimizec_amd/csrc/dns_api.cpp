@@ -42,10 +42,13 @@ const char *const kKernelNames[KT_COUNT] = {"k_row", "k_col", "k_row_cfl"};
 struct dns_handle {
     int N = 0, M = 0, H = 0, NK = 0;
     int device = 0;
+    int fp32 = 0;               // 1: V, G and the transforms in FP32 (state stays FP64)
     dns_scalars sc{};
     hipStream_t stream = nullptr;
-    cd *om = nullptr, *fn = nullptr, *V = nullptr, *G = nullptr, *tw = nullptr;
-    cd *Vd = nullptr;           // diagnostics: 2 z-inverted fields (lazy)
+    cd *om = nullptr, *fn = nullptr, *tw = nullptr;
+    void *V = nullptr, *G = nullptr;  // transform precision (cd or cf)
+    void *twT = nullptr;        // twiddles in the transform precision (== tw in FP64)
+    cd *Vd = nullptr;           // diagnostics (FP64): 2 z-inverted fields; host staging (lazy)
     double *phys = nullptr;     // diagnostics: one M x M physical plane (lazy)
     double *cflrow = nullptr;   // per-row CFL maxima (M)
     double *partEZ = nullptr;   // per-column E, Z partials (2 H)
@@ -164,23 +167,29 @@ dns_status refresh_v(dns_handle *h)
     static const int f4[4] = {DNS_F_U, DNS_F_W, DNS_F_OX, DNS_F_OZ};
     LaunchReq r{};
     r.what = LAUNCH_COL;
+    r.fp32 = h->fp32;
     r.om = h->om;
     r.fn = h->fn;
     r.G = h->G;
     r.V = h->V;
     r.tw = h->tw;
+    r.twT = h->twT;
     r.col = inverse_args(h, 4, f4, (size_t)h->M * h->H);
     dns_status st = launch(h, -1, r);
     if (st == DNS_OK) h->v_valid = true;
     return st;
 }
 
+dns_status ensure_vd(dns_handle *h)
+{
+    if (!h->Vd) return dev_alloc(h, (void **)&h->Vd, sizeof(cd) * 2 * (size_t)h->M * h->H, "diag spectra");
+    return DNS_OK;
+}
+
 dns_status ensure_diag_buffers(dns_handle *h)
 {
-    if (!h->Vd) {
-        dns_status st = dev_alloc(h, (void **)&h->Vd, sizeof(cd) * 2 * (size_t)h->M * h->H, "diag spectra");
-        if (st != DNS_OK) return st;
-    }
+    dns_status st = ensure_vd(h);
+    if (st != DNS_OK) return st;
     if (!h->phys) {
         dns_status st = dev_alloc(h, (void **)&h->phys, sizeof(double) * (size_t)h->M * h->M, "physical plane");
         if (st != DNS_OK) return st;
@@ -210,16 +219,19 @@ dns_status compute_field(dns_handle *h, int field)
     const size_t vs = (size_t)h->M * h->H;
     LaunchReq r{};
     r.what = LAUNCH_COL;
+    r.fp32 = 0;  // diagnostics run in FP64 in every precision mode
     r.om = h->om;
     r.fn = h->fn;
-    r.G = h->G;
+    r.G = nullptr;
     r.V = h->Vd;
     r.tw = h->tw;
+    r.twT = h->tw;
     r.col = inverse_args(h, nf, fields, vs);
     st = launch(h, -1, r);
     if (st != DNS_OK) return st;
     LaunchReq p{};
     p.what = LAUNCH_ROW_PHYS;
+    p.fp32 = 0;
     p.tw = h->tw;
     p.Va = h->Vd;
     p.Vb = (nf == 2) ? h->Vd + vs : nullptr;
@@ -229,22 +241,46 @@ dns_status compute_field(dns_handle *h, int field)
     return launch(h, -1, p);
 }
 
+// CFLM of the current state.  FP64 mode reuses the step's z-inverted u, w (V); in
+// FP32 mode u, w are z-inverted again in FP64 into the diagnostics buffer, so the
+// CFL/dt logic always sees FP64 velocities.
 dns_status compute_cflm(dns_handle *h, double *cflm)
 {
-    if (!h->v_valid) {
-        dns_status st = refresh_v(h);
-        if (st != DNS_OK) return st;
-    }
     const double PI = 3.14159265358979323846;
+    const size_t vs = (size_t)h->M * h->H;
     LaunchReq r{};
     r.what = LAUNCH_ROW_CFL;
-    r.V = h->V;
-    r.vstride = (size_t)h->M * h->H;
+    r.fp32 = 0;
+    r.vstride = vs;
     r.Gw = nullptr;
     r.tw = h->tw;
+    r.twT = h->tw;
     r.cflrow = h->cflrow;
     r.DX = 2.0 * PI / double(h->N);
     r.DZ = 2.0 * PI / double(h->N);
+    if (!h->fp32) {
+        if (!h->v_valid) {
+            dns_status st = refresh_v(h);
+            if (st != DNS_OK) return st;
+        }
+        r.V = h->V;
+    } else {
+        dns_status st = ensure_vd(h);
+        if (st != DNS_OK) return st;
+        static const int f2[2] = {DNS_F_U, DNS_F_W};
+        LaunchReq c{};
+        c.what = LAUNCH_COL;
+        c.fp32 = 0;
+        c.om = h->om;
+        c.fn = h->fn;
+        c.V = h->Vd;
+        c.tw = h->tw;
+        c.twT = h->tw;
+        c.col = inverse_args(h, 2, f2, vs);
+        st = launch(h, -1, c);
+        if (st != DNS_OK) return st;
+        r.V = h->Vd;
+    }
     dns_status st = launch(h, KT_ROW_CFL, r);
     if (st != DNS_OK) return st;
     std::vector<double> rows(h->M);
@@ -256,12 +292,14 @@ dns_status compute_cflm(dns_handle *h, double *cflm)
     return DNS_OK;
 }
 
-// host [iz][ix] <-> device [ix][iz] through the G buffer as staging
+// host [iz][ix] <-> device [ix][iz] through the diagnostics buffer as staging
 dns_status upload_spec(dns_handle *h, const double *host, cd *dst)
 {
     const size_t n = (size_t)h->N * h->H;
-    HIPCHK(h, hipMemcpyAsync(h->G, host, sizeof(cd) * n, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, dns_launch_transpose(h->G, dst, h->N, h->H, h->stream));
+    dns_status st = ensure_vd(h);
+    if (st != DNS_OK) return st;
+    HIPCHK(h, hipMemcpyAsync(h->Vd, host, sizeof(cd) * n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, dns_launch_transpose(h->Vd, dst, h->N, h->H, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return DNS_OK;
 }
@@ -269,8 +307,10 @@ dns_status upload_spec(dns_handle *h, const double *host, cd *dst)
 dns_status download_spec(dns_handle *h, const cd *src, double *host)
 {
     const size_t n = (size_t)h->N * h->H;
-    HIPCHK(h, dns_launch_transpose(src, h->G, h->H, h->N, h->stream));
-    HIPCHK(h, hipMemcpyAsync(host, h->G, sizeof(cd) * n, hipMemcpyDeviceToHost, h->stream));
+    dns_status st = ensure_vd(h);
+    if (st != DNS_OK) return st;
+    HIPCHK(h, dns_launch_transpose(src, h->Vd, h->H, h->N, h->stream));
+    HIPCHK(h, hipMemcpyAsync(host, h->Vd, sizeof(cd) * n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return DNS_OK;
 }
@@ -309,8 +349,10 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     if (cfg->N < 4 || cfg->N % 4 != 0) return fail(nullptr, DNS_E_ARG, "N=%d must be a positive multiple of 4", cfg->N);
     if (!dns_supported_n(cfg->N))
         return fail(nullptr, DNS_E_UNSUPPORTED, "N=%d: no kernels built for H=N/2=%d", cfg->N, cfg->N / 2);
-    if (cfg->precision != DNS_PREC_FP64)
+    if (cfg->precision != DNS_PREC_FP64 && cfg->precision != DNS_PREC_FP32)
         return fail(nullptr, DNS_E_UNSUPPORTED, "precision %d is not built", cfg->precision);
+    if (cfg->precision == DNS_PREC_FP32 && cfg->N < 128)
+        return fail(nullptr, DNS_E_UNSUPPORTED, "FP32 transforms are built for N >= 128");
     if (!(cfg->Re > 0.0)) return fail(nullptr, DNS_E_ARG, "Re must be positive");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
@@ -324,6 +366,7 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     h->M = 3 * cfg->N / 2;
     h->NK = 3 * cfg->N / 4 + 1;
     h->device = cfg->device;
+    h->fp32 = cfg->precision == DNS_PREC_FP32 ? 1 : 0;
     h->sc.N = h->N;
     h->sc.M = h->M;
     h->sc.H = h->H;
@@ -349,8 +392,9 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     dns_status st;
     if ((st = dev_alloc(h, (void **)&h->om, sizeof(cd) * nspec, "om2")) != DNS_OK) return bail(st);
     if ((st = dev_alloc(h, (void **)&h->fn, sizeof(cd) * nspec, "fnm1")) != DNS_OK) return bail(st);
-    if ((st = dev_alloc(h, (void **)&h->V, sizeof(cd) * 4 * nmid, "z-inverse fields")) != DNS_OK) return bail(st);
-    if ((st = dev_alloc(h, (void **)&h->G, sizeof(cd) * nmid, "x-forward product")) != DNS_OK) return bail(st);
+    const size_t elem = h->fp32 ? sizeof(dnsfft::cf) : sizeof(cd);
+    if ((st = dev_alloc(h, &h->V, elem * 4 * nmid, "z-inverse fields")) != DNS_OK) return bail(st);
+    if ((st = dev_alloc(h, &h->G, elem * nmid, "x-forward product")) != DNS_OK) return bail(st);
     if ((st = dev_alloc(h, (void **)&h->tw, sizeof(cd) * h->M, "twiddles")) != DNS_OK) return bail(st);
     if ((st = dev_alloc(h, (void **)&h->cflrow, sizeof(double) * h->M, "cfl rows")) != DNS_OK) return bail(st);
     if ((st = dev_alloc(h, (void **)&h->partEZ, sizeof(double) * 2 * h->H, "energy partials")) != DNS_OK)
@@ -369,6 +413,14 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     }
     if ((e = hipMemcpy(h->tw, tw.data(), sizeof(cd) * h->M, hipMemcpyHostToDevice)) != hipSuccess)
         return bail(fail(h, DNS_E_HIP, "twiddle upload: %s", hipGetErrorString(e)));
+    h->twT = h->tw;
+    if (h->fp32) {
+        std::vector<dnsfft::cf> twf(h->M);
+        for (int t = 0; t < h->M; ++t) twf[t] = dnsfft::cf{float(tw[t].x), float(tw[t].y)};
+        if ((st = dev_alloc(h, &h->twT, sizeof(dnsfft::cf) * h->M, "fp32 twiddles")) != DNS_OK) return bail(st);
+        if ((e = hipMemcpy(h->twT, twf.data(), sizeof(dnsfft::cf) * h->M, hipMemcpyHostToDevice)) != hipSuccess)
+            return bail(fail(h, DNS_E_HIP, "twiddle upload: %s", hipGetErrorString(e)));
+    }
     if ((e = hipMemset(h->om, 0, sizeof(cd) * nspec)) != hipSuccess ||
         (e = hipMemset(h->fn, 0, sizeof(cd) * nspec)) != hipSuccess)
         return bail(fail(h, DNS_E_HIP, "state clear: %s", hipGetErrorString(e)));
@@ -386,7 +438,8 @@ extern "C" void dns_destroy(dns_handle *h)
         (void)hipEventDestroy(t.b);
     }
     for (hipEvent_t ev : h->event_pool) (void)hipEventDestroy(ev);
-    void *ptrs[] = {h->om, h->fn, h->V, h->G, h->tw, h->Vd, h->phys, h->cflrow, h->partEZ, h->red, h->redu};
+    if (h->twT == h->tw) h->twT = nullptr;
+    void *ptrs[] = {h->om, h->fn, h->V, h->G, h->tw, h->twT, h->Vd, h->phys, h->cflrow, h->partEZ, h->red, h->redu};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -447,21 +500,25 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
         const double dt = h->sc.dt;
         LaunchReq r{};
         r.what = LAUNCH_ROW;
+        r.fp32 = h->fp32;
         r.V = h->V;
         r.vstride = vs;
         r.Gw = h->G;
         r.tw = h->tw;
+        r.twT = h->twT;
         r.cflrow = nullptr;
         r.DX = r.DZ = 2.0 * PI / double(h->N);
         dns_status st = launch(h, KT_ROW, r);
         if (st != DNS_OK) return st;
         LaunchReq c{};
         c.what = LAUNCH_COL;
+        c.fp32 = h->fp32;
         c.om = h->om;
         c.fn = h->fn;
         c.G = h->G;
         c.V = h->V;
         c.tw = h->tw;
+        c.twT = h->twT;
         c.col.do_forward = 1;
         c.col.nfields = 4;
         for (int i = 0; i < 4; ++i) c.col.fields[i] = f4[i];
@@ -813,11 +870,13 @@ extern "C" dns_status dns_kernel_timing(dns_handle *h, const char **names, int64
     if (!h || !n) return DNS_E_ARG;
     drain_timing(h);
     const double N2 = double(h->N) * double(h->N);
-    // algorithmic HBM bytes per launch (FP64 complex = 16 B):
-    //   k_row     reads 4 fields M x H, writes 1: 5 * 1.5 N * 0.5 N * 16 = 60 N^2
-    //   k_col     reads G (12 N^2) + om, fn (16 N^2), writes om, fn (16 N^2) + V (48 N^2) = 92 N^2
-    //   k_row_cfl reads 2 fields: 24 N^2
-    const double bytes[KT_COUNT] = {60.0 * N2, 92.0 * N2, 24.0 * N2};
+    // algorithmic HBM bytes per launch, with c = bytes of a transform-precision complex
+    // (16 in FP64, 8 in FP32) and 16 for the FP64 state om, fn:
+    //   k_row     reads 4 fields M x H, writes 1: 5 * 0.75 N^2 * c           (60 N^2 | 30 N^2)
+    //   k_col     reads G + om, fn, writes om, fn + V: 3.75 c N^2 + 32 N^2   (92 N^2 | 62 N^2)
+    //   k_row_cfl reads u, w in FP64: 24 N^2
+    const double c = h->fp32 ? 8.0 : 16.0;
+    const double bytes[KT_COUNT] = {3.75 * c * N2, (3.75 * c + 32.0) * N2, 24.0 * N2};
     const int cnt = std::min(*n, (int)KT_COUNT);
     for (int i = 0; i < cnt; ++i) {
         if (names) names[i] = kKernelNames[i];

@@ -33,15 +33,17 @@ enum { LAUNCH_COL = 0, LAUNCH_ROW = 1, LAUNCH_ROW_CFL = 2, LAUNCH_ROW_PHYS = 3 }
 struct LaunchReq {
     int what;
     hipStream_t stream;
+    int fp32;                  // 1: transforms and V, G in FP32 (cf); 0: FP64 (cd)
     // column kernel
     dnsfft::cd *om, *fn;
-    const dnsfft::cd *G;
-    dnsfft::cd *V;
-    const dnsfft::cd *tw;
+    const void *G;             // cd or cf
+    void *V;                   // cd or cf
+    const dnsfft::cd *tw;      // FP64 twiddles W_M^t
+    const void *twT;           // twiddles of the transform precision
     ColArgs col;
     // row kernels
     size_t vstride;
-    dnsfft::cd *Gw;
+    void *Gw;
     double *cflrow;
     double DX, DZ;
     const dnsfft::cd *Va, *Vb;

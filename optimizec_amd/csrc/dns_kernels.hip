@@ -98,17 +98,17 @@ __device__ __forceinline__ double inv_g_of(int iz, double gz) { return (iz >= 1 
 //             W_M^{-z2 k1} (S[k1] + S[k1+2H] W_3^{-2 z2}), inverse-transformed and written
 //             to rows z = z2 + 3 z1 of V.
 // =====================================================================================
-template <int H, int C, int T>
-__global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
-                                           const cd *__restrict__ G, cd *__restrict__ V,
-                                           const cd *__restrict__ tw, ColArgs a)
+template <class TC, int H, int C, int T, int MINW>
+__global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
+                                           const TC *__restrict__ G, TC *__restrict__ V,
+                                           const TC *__restrict__ twT, const cd *__restrict__ tw, ColArgs a)
 {
     constexpr int N = 2 * H;
     constexpr int M = 3 * H;
     constexpr int PJ = C * H / T;
     constexpr int LS = padded_len(H);
     static_assert((C * H) % T == 0, "tile must divide evenly over the threads");
-    __shared__ cd lds[3 * C * LS];
+    __shared__ TC lds[3 * C * LS];
 
     const int tid = threadIdx.x;
     const int kx0 = xcd_tile(blockIdx.x, gridDim.x) * C;
@@ -123,16 +123,17 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
             lds[(z2 * C + c) * LS + pidx(z1)] = G[(size_t)z * H + kx0 + c];
         }
         __syncthreads();
-        fft_lines<H, LS, 3 * C, T, false>(lds, tw, 3);
+        fft_lines<H, LS, 3 * C, T, false>(lds, twT, 3);
+        // radix-3 combine in FP64 (the FFT outputs are widened)
         const cd w3a = tw[2 * H];  // W_3^2
         const cd w3b = tw[H];      // W_3^4 = W_3
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
             const int p = tid + j * T;
             const int c = p / H, k1 = p - (p / H) * H;
-            const cd F0 = lds[(0 * C + c) * LS + pidx(k1)];
-            const cd F1 = mul(lds[(1 * C + c) * LS + pidx(k1)], tw[opaque(k1)]);
-            const cd F2 = mul(lds[(2 * C + c) * LS + pidx(k1)], tw[opaque(2 * k1)]);
+            const cd F0 = to_d(lds[(0 * C + c) * LS + pidx(k1)]);
+            const cd F1 = mul(to_d(lds[(1 * C + c) * LS + pidx(k1)]), tw[opaque(k1)]);
+            const cd F2 = mul(to_d(lds[(2 * C + c) * LS + pidx(k1)]), tw[opaque(2 * k1)]);
             xlo[j] = add(F0, add(F1, F2));
             xhi[j] = add(F0, add(mul(F1, w3a), mul(F2, w3b)));
         }
@@ -209,13 +210,13 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
             const cd slo = field_spec(f, xlo[j], kx, k1, ax, glo, ik_lo[j], iglo);
             const cd shi = field_spec(f, xhi[j], kx, k1 + H, ax, ghi, ik_hi[j], ighi);
             // residue z2: W_M^{-z2 k1} (S[k1] + S[k1 + 2H] W_3^{-2 z2})
-            lds[(0 * C + c) * LS + pidx(k1)] = add(slo, shi);
-            lds[(1 * C + c) * LS + pidx(k1)] = mulc(add(slo, mulc(shi, tw[2 * H])), tw[opaque(k1)]);
-            lds[(2 * C + c) * LS + pidx(k1)] = mulc(add(slo, mulc(shi, tw[H])), tw[opaque(2 * k1)]);
+            lds[(0 * C + c) * LS + pidx(k1)] = from_d<TC>(add(slo, shi));
+            lds[(1 * C + c) * LS + pidx(k1)] = from_d<TC>(mulc(add(slo, mulc(shi, tw[2 * H])), tw[opaque(k1)]));
+            lds[(2 * C + c) * LS + pidx(k1)] = from_d<TC>(mulc(add(slo, mulc(shi, tw[H])), tw[opaque(2 * k1)]));
         }
         __syncthreads();
-        fft_lines<H, LS, 3 * C, T, true>(lds, tw, 3);
-        cd *out = V + (size_t)fi * a.vstride + (size_t)kx0 * M;
+        fft_lines<H, LS, 3 * C, T, true>(lds, twT, 3);
+        TC *out = V + (size_t)fi * a.vstride + (size_t)kx0 * M;
         for (int e = tid; e < C * M; e += T) {
             const int c = e / M, zz = e - c * M;
             const int z1 = zz / 3, z2 = zz - 3 * z1;
@@ -239,17 +240,18 @@ __global__ __launch_bounds__(T) void k_col(cd *__restrict__ om, cd *__restrict__
 // wave hold the pair (q, H - q), and one cross-lane exchange gives each its mirror.
 // The three residues of one field pair are transformed together (3 lines).
 // =====================================================================================
-template <int H, int T, int MODE>
-__global__ __launch_bounds__(T) void k_row(const cd *__restrict__ V, size_t vstride, cd *__restrict__ G,
-                                           const cd *__restrict__ tw, double *__restrict__ cflrow,
+template <class TC, int H, int T, int MODE, int MINW>
+__global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_t vstride, TC *__restrict__ G,
+                                           const TC *__restrict__ tw, double *__restrict__ cflrow,
                                            double DX, double DZ)
 {
+    using RT = decltype(TC::x);
     constexpr int LS = padded_len(H);
     constexpr int PJ = (H / T) > 0 ? (H / T) : 1;  // input pairs per lane
     constexpr int PO = (H + T - 1) / T;            // outputs per thread and residue
     static_assert(T % 64 == 0, "whole wavefronts");
     static_assert(H % T == 0 || H < T, "pair slots must tile the half row");
-    __shared__ cd lds[3 * LS];
+    __shared__ TC lds[3 * LS];
     __shared__ double red[T / 64];
     constexpr int M = 3 * H;
     const int tid = threadIdx.x;
@@ -257,44 +259,45 @@ __global__ __launch_bounds__(T) void k_row(const cd *__restrict__ V, size_t vstr
     // rows in XCD-aware order: neighbouring rows share the 128-byte lines of the
     // column-major V, so they should meet in one L2
     const int z = xcd_tile(blockIdx.x, gridDim.x);
-    const cd *row = V + z;  // element (f, k) at f * vstride + k * M
+    const TC *row = V + z;  // element (f, k) at f * vstride + k * M
     // W_3^{2 n2} for n2 = 0, 1, 2 (indices (2 n2 H) mod M)
-    const cd w3_1 = tw[2 * H], w3_2 = tw[H];
+    const TC w3_1 = tw[2 * H], w3_2 = tw[H];
+    const TC zero{RT(0), RT(0)};
 
-    cd vel[3][PO];     // u + i w at x = n2 + 3 x1
-    double P[3][PO];   // u d_x omega + w d_z omega
+    TC vel[3][PO];     // u + i w at x = n2 + 3 x1
+    RT P[3][PO];       // u d_x omega + w d_z omega
     double cmax = 0.0;
 
 #pragma unroll 1
     for (int ph = 0; ph < ((MODE == 0) ? 2 : 1); ++ph) {
-        const cd *fa = row + (size_t)(2 * ph) * vstride;
-        const cd *fb = fa + vstride;
+        const TC *fa = row + (size_t)(2 * ph) * vstride;
+        const TC *fb = fa + vstride;
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
             const int q = (tid >> 6) * 32 + (lane & 31) + j * (T / 2);
             const bool valid = q < H / 2;
             const bool hi = lane >= 32;
             const int k = !hi ? q : (q == 0 ? H / 2 : H - q);
-            cd av = mk(0.0, 0.0), bv = mk(0.0, 0.0);
+            TC av = zero, bv = zero;
             if (valid) {
                 av = fa[(size_t)k * M];
                 bv = fb[(size_t)k * M];
             }
             // partner lane (l ^ 32) holds H - k, except in the q = 0 pair (0 and H/2)
-            cd am = mk(__shfl_xor(av.x, 32, 64), __shfl_xor(av.y, 32, 64));
-            cd bm = mk(__shfl_xor(bv.x, 32, 64), __shfl_xor(bv.y, 32, 64));
+            TC am{__shfl_xor(av.x, 32, 64), __shfl_xor(av.y, 32, 64)};
+            TC bm{__shfl_xor(bv.x, 32, 64), __shfl_xor(bv.y, 32, 64)};
             if (k == H / 2) {  // its own mirror
                 am = av;
                 bm = bv;
             }
             if (valid) {
-                cd zlo, zhi;
+                TC zlo, zhi;
                 if (k == 0) {
-                    zlo = mk(av.x, bv.x);
-                    zhi = mk(0.0, 0.0);
+                    zlo = TC{av.x, bv.x};
+                    zhi = zero;
                 } else {
-                    zlo = mk(av.x - bv.y, av.y + bv.x);
-                    zhi = mk(am.x + bm.y, bm.x - am.y);
+                    zlo = TC{av.x - bv.y, av.y + bv.x};
+                    zhi = TC{am.x + bm.y, bm.x - am.y};
                 }
                 lds[pidx(k)] = add(zlo, zhi);
                 lds[LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_1)), tw[opaque(k)]);
@@ -309,11 +312,11 @@ __global__ __launch_bounds__(T) void k_row(const cd *__restrict__ V, size_t vstr
             if (x1 < H) {
 #pragma unroll
                 for (int n2 = 0; n2 < 3; ++n2) {
-                    const cd v = lds[n2 * LS + pidx(x1)];
+                    const TC v = lds[n2 * LS + pidx(x1)];
                     if (ph == 0) {
                         vel[n2][j] = v;
-                        const double cf = fabs(v.x) / DX + fabs(v.y) / DZ;
-                        cmax = cf > cmax ? cf : cmax;
+                        const double cfl = fabs(double(v.x)) / DX + fabs(double(v.y)) / DZ;
+                        cmax = cfl > cmax ? cfl : cmax;
                     } else {
                         P[n2][j] = vel[n2][j].x * v.x + vel[n2][j].y * v.y;
                     }
@@ -326,26 +329,27 @@ __global__ __launch_bounds__(T) void k_row(const cd *__restrict__ V, size_t vstr
     if (MODE == 0) {
         // x-forward of the real product: F_n2 = H-point FFT of P[n2 + 3 x1] (lines
         // A = P0 + i P1, B = P2), then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
-        cd *A = lds;
-        cd *B = lds + LS;
+        TC *A = lds;
+        TC *B = lds + LS;
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
             if (x1 < H) {
-                A[pidx(x1)] = mk(P[0][j], P[1][j]);
-                B[pidx(x1)] = mk(P[2][j], 0.0);
+                A[pidx(x1)] = TC{P[0][j], P[1][j]};
+                B[pidx(x1)] = TC{P[2][j], RT(0)};
             }
         }
         __syncthreads();
         fft_lines<H, LS, 2, T, false>(lds, tw, 3);
-        cd *g = G + (size_t)z * H;
+        TC *g = G + (size_t)z * H;
+        const RT half = RT(0.5);
         for (int k = tid; k < H; k += T) {
-            const cd za = A[pidx(k)];
-            const cd zm = conj(A[pidx(k == 0 ? 0 : H - k)]);
-            const cd s = add(za, zm), d = sub(za, zm);
-            const cd F0 = mk(0.5 * s.x, 0.5 * s.y);
-            const cd F1 = mk(0.5 * d.y, -0.5 * d.x);  // d / (2 i)
-            const cd F2 = B[pidx(k)];
+            const TC za = A[pidx(k)];
+            const TC zm = conj(A[pidx(k == 0 ? 0 : H - k)]);
+            const TC s = add(za, zm), d = sub(za, zm);
+            const TC F0{half * s.x, half * s.y};
+            const TC F1{half * d.y, -half * d.x};  // d / (2 i)
+            const TC F2 = B[pidx(k)];
             g[k] = add(F0, add(mul(F1, tw[opaque(k)]), mul(F2, tw[opaque(2 * k)])));
         }
     }
@@ -398,7 +402,7 @@ __global__ __launch_bounds__(T) void k_row_phys(const cd *__restrict__ Va, const
             A[pidx(k1)] = mulc(add(zlo, mulc(zhi, w3)), tw[opaque(x2 * k1)]);
         }
         __syncthreads();
-        fft_lines<H, LS, 1, T, true>(A, tw, 3);
+        fft_lines<H, LS, 1, T, true, cd>(A, tw, 3);
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
             const int x1 = tid + j * T;
@@ -558,23 +562,37 @@ DNS_CFG(1536, 1, 768, 768)
 DNS_CFG(2048, 1, 512, 512)
 #undef DNS_CFG
 
-template <int H>
-hipError_t launch_all(const LaunchReq &r)
+// Minimum waves per SIMD the compiler budgets VGPRs for.  Forcing two workgroups per
+// CU (FP32 lines at H >= 1024 fit twice in LDS) made the kernels spill and run slower
+// on MI355X (row 0.71 -> 0.76 ms, column 0.45 -> 0.55 ms at N=4096), so it stays 1.
+template <class TC, int H, int T, int LDS_BYTES>
+constexpr int min_waves()
+{
+    return 1;
+}
+
+template <class TC, int H>
+hipError_t launch_prec(const LaunchReq &r)
 {
     using Q = Cfg<H>;
     constexpr int M = 3 * H;
+    constexpr int WC = min_waves<TC, H, Q::TC, 3 * Q::C * padded_len(H) * (int)sizeof(TC)>();
+    constexpr int WR = min_waves<TC, H, Q::TR, 3 * padded_len(H) * (int)sizeof(TC)>();
+    const TC *twT = static_cast<const TC *>(r.twT);
     switch (r.what) {
         case LAUNCH_COL:
-            hipLaunchKernelGGL((k_col<H, Q::C, Q::TC>), dim3(H / Q::C), dim3(Q::TC), 0, r.stream, r.om, r.fn,
-                               r.G, r.V, r.tw, r.col);
+            hipLaunchKernelGGL((k_col<TC, H, Q::C, Q::TC, WC>), dim3(H / Q::C), dim3(Q::TC), 0, r.stream, r.om, r.fn,
+                               static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col);
             break;
         case LAUNCH_ROW:
-            hipLaunchKernelGGL((k_row<H, Q::TR, 0>), dim3(M), dim3(Q::TR), 0, r.stream, r.V, r.vstride, r.Gw,
-                               r.tw, r.cflrow, r.DX, r.DZ);
+            hipLaunchKernelGGL((k_row<TC, H, Q::TR, 0, WR>), dim3(M), dim3(Q::TR), 0, r.stream,
+                               static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
+                               r.DX, r.DZ);
             break;
         case LAUNCH_ROW_CFL:
-            hipLaunchKernelGGL((k_row<H, Q::TR, 1>), dim3(M), dim3(Q::TR), 0, r.stream, r.V, r.vstride, r.Gw,
-                               r.tw, r.cflrow, r.DX, r.DZ);
+            hipLaunchKernelGGL((k_row<TC, H, Q::TR, 1, WR>), dim3(M), dim3(Q::TR), 0, r.stream,
+                               static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
+                               r.DX, r.DZ);
             break;
         case LAUNCH_ROW_PHYS:
             hipLaunchKernelGGL((k_row_phys<H, Q::TR>), dim3(M), dim3(Q::TR), 0, r.stream, r.Va, r.Vb, r.tw,
@@ -584,6 +602,16 @@ hipError_t launch_all(const LaunchReq &r)
             return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int H>
+hipError_t launch_all(const LaunchReq &r)
+{
+    if (r.fp32) {
+        if constexpr (H >= 64) return launch_prec<cf, H>(r);
+        return hipErrorInvalidValue;
+    }
+    return launch_prec<cd, H>(r);
 }
 
 }  // namespace

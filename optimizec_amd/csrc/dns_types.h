@@ -8,4 +8,9 @@ struct cd {
     double x, y;
 };
 
+// FP32 complex value (8 bytes) of the mixed-precision transforms
+struct cf {
+    float x, y;
+};
+
 }  // namespace dnsfft

@@ -1,7 +1,7 @@
 // fft_lds.hpp -- LDS-resident Stockham FFT engine for gfx950 (wave64).
 //
 // A workgroup transforms NL lines of length L held in LDS (line stride LS complex
-// doubles).  Each radix-R stage: every thread loads the R inputs of its butterflies
+// values; FP64 `cd` or FP32 `cf`).  Each radix-R stage: every thread loads the R inputs of its butterflies
 // from LDS into VGPRs (twiddled), the workgroup barriers, the thread runs the radix-R
 // DFT in registers and writes the R outputs to their Stockham (self-sorting)
 // positions, barrier.  The radix sequence is fixed at compile time (8, then 4, 2, 3,
@@ -19,24 +19,44 @@
 namespace dnsfft {
 
 __device__ __forceinline__ cd mk(double a, double b) { return cd{a, b}; }
-__device__ __forceinline__ cd add(cd a, cd b) { return cd{a.x + b.x, a.y + b.y}; }
-__device__ __forceinline__ cd sub(cd a, cd b) { return cd{a.x - b.x, a.y - b.y}; }
-__device__ __forceinline__ cd mul(cd a, cd b)
+__device__ __forceinline__ cf mkf(float a, float b) { return cf{a, b}; }
+template <class C>
+__device__ __forceinline__ C add(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <class C>
+__device__ __forceinline__ C sub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <class C>
+__device__ __forceinline__ C mul(C a, C b)
 {
-    return cd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
-__device__ __forceinline__ cd mulc(cd a, cd b)  // a * conj(b)
+template <class C>
+__device__ __forceinline__ C mulc(C a, C b)  // a * conj(b)
 {
-    return cd{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+    return C{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
 }
-__device__ __forceinline__ cd scl(cd a, double s) { return cd{a.x * s, a.y * s}; }
-__device__ __forceinline__ cd conj(cd a) { return cd{a.x, -a.y}; }
+template <class C, class S>
+__device__ __forceinline__ C scl(C a, S s)
+{
+    using RT = decltype(a.x);
+    return C{a.x * RT(s), a.y * RT(s)};
+}
+template <class C>
+__device__ __forceinline__ C conj(C a) { return C{a.x, -a.y}; }
 // multiply by +i (S>0) or -i (S<0)
-template <int S>
-__device__ __forceinline__ cd muli(cd a)
+template <int S, class C>
+__device__ __forceinline__ C muli(C a)
 {
-    return S > 0 ? cd{-a.y, a.x} : cd{a.y, -a.x};
+    return S > 0 ? C{-a.y, a.x} : C{a.y, -a.x};
 }
+__device__ __forceinline__ cf to_f(cd a) { return cf{float(a.x), float(a.y)}; }
+__device__ __forceinline__ cd to_d(cf a) { return cd{double(a.x), double(a.y)}; }
+__device__ __forceinline__ cd to_d(cd a) { return a; }
+template <class C>
+__device__ __forceinline__ C from_d(cd a);
+template <>
+__device__ __forceinline__ cd from_d<cd>(cd a) { return a; }
+template <>
+__device__ __forceinline__ cf from_d<cf>(cd a) { return to_f(a); }
 
 // Returns x through an empty asm statement: table loads indexed by the result cannot
 // be hoisted out of loops by the compiler (twiddles are loop-invariant in many callers,
@@ -97,30 +117,31 @@ __host__ __device__ constexpr bool smooth7(int L)
 
 // ---- small DFTs in registers ----------------------------------------------------
 // Constants are the exact values cos/sin(2 pi q / R) to double precision.
-template <int R, bool INV>
-__device__ __forceinline__ void dft(cd *v)
+template <int R, bool INV, class C>
+__device__ __forceinline__ void dft(C *v)
 {
+    using RT = decltype(v[0].x);
     constexpr int S = INV ? 1 : -1;  // sign of the exponent
     if constexpr (R == 2) {
-        const cd a = v[0], b = v[1];
+        const C a = v[0], b = v[1];
         v[0] = add(a, b);
         v[1] = sub(a, b);
     } else if constexpr (R == 4) {
-        const cd s02 = add(v[0], v[2]), d02 = sub(v[0], v[2]);
-        const cd s13 = add(v[1], v[3]), d13 = muli<S>(sub(v[1], v[3]));
+        const C s02 = add(v[0], v[2]), d02 = sub(v[0], v[2]);
+        const C s13 = add(v[1], v[3]), d13 = muli<S>(sub(v[1], v[3]));
         v[0] = add(s02, s13);
         v[1] = add(d02, d13);
         v[2] = sub(s02, s13);
         v[3] = sub(d02, d13);
     } else if constexpr (R == 8) {
-        cd e[4] = {v[0], v[2], v[4], v[6]};
-        cd o[4] = {v[1], v[3], v[5], v[7]};
-        dft<4, INV>(e);
-        dft<4, INV>(o);
+        C e[4] = {v[0], v[2], v[4], v[6]};
+        C o[4] = {v[1], v[3], v[5], v[7]};
+        dft<4, INV, C>(e);
+        dft<4, INV, C>(o);
         constexpr double h = 0.70710678118654752440084436210484903928;
-        const cd t1 = mul(o[1], cd{h, S * h});
-        const cd t2 = muli<S>(o[2]);
-        const cd t3 = mul(o[3], cd{-h, S * h});
+        const C t1 = mul(o[1], C{RT(h), RT(S * h)});
+        const C t2 = muli<S>(o[2]);
+        const C t3 = mul(o[3], C{RT(-h), RT(S * h)});
         v[0] = add(e[0], o[0]);
         v[4] = sub(e[0], o[0]);
         v[1] = add(e[1], t1);
@@ -132,9 +153,9 @@ __device__ __forceinline__ void dft(cd *v)
     } else if constexpr (R == 3) {
         constexpr double c1 = -0.5;
         constexpr double s1 = 0.86602540378443864676372317075293618347;
-        const cd sp = add(v[1], v[2]), dm = sub(v[1], v[2]);
-        const cd t = add(v[0], scl(sp, c1));
-        const cd u = muli<S>(scl(dm, s1));
+        const C sp = add(v[1], v[2]), dm = sub(v[1], v[2]);
+        const C t = add(v[0], scl(sp, c1));
+        const C u = muli<S>(scl(dm, s1));
         v[0] = add(v[0], sp);
         v[1] = add(t, u);
         v[2] = sub(t, u);
@@ -143,12 +164,12 @@ __device__ __forceinline__ void dft(cd *v)
         constexpr double c2 = -0.80901699437494742410229341718281905886;
         constexpr double s1 = 0.95105651629515357211643933337938214340;
         constexpr double s2 = 0.58778525229247312916870595463907276860;
-        const cd sp1 = add(v[1], v[4]), dm1 = sub(v[1], v[4]);
-        const cd sp2 = add(v[2], v[3]), dm2 = sub(v[2], v[3]);
-        const cd t1 = add(v[0], add(scl(sp1, c1), scl(sp2, c2)));
-        const cd t2 = add(v[0], add(scl(sp1, c2), scl(sp2, c1)));
-        const cd u1 = muli<S>(add(scl(dm1, s1), scl(dm2, s2)));
-        const cd u2 = muli<S>(sub(scl(dm1, s2), scl(dm2, s1)));
+        const C sp1 = add(v[1], v[4]), dm1 = sub(v[1], v[4]);
+        const C sp2 = add(v[2], v[3]), dm2 = sub(v[2], v[3]);
+        const C t1 = add(v[0], add(scl(sp1, c1), scl(sp2, c2)));
+        const C t2 = add(v[0], add(scl(sp1, c2), scl(sp2, c1)));
+        const C u1 = muli<S>(add(scl(dm1, s1), scl(dm2, s2)));
+        const C u2 = muli<S>(sub(scl(dm1, s2), scl(dm2, s1)));
         v[0] = add(v[0], add(sp1, sp2));
         v[1] = add(t1, u1);
         v[4] = sub(t1, u1);
@@ -161,16 +182,16 @@ __device__ __forceinline__ void dft(cd *v)
         constexpr double s1 = 0.78183148246802980870844452667405775023;
         constexpr double s2 = 0.97492791218182360701813168299393121723;
         constexpr double s3 = 0.43388373911755812047576833284835875461;
-        const cd sp1 = add(v[1], v[6]), dm1 = sub(v[1], v[6]);
-        const cd sp2 = add(v[2], v[5]), dm2 = sub(v[2], v[5]);
-        const cd sp3 = add(v[3], v[4]), dm3 = sub(v[3], v[4]);
+        const C sp1 = add(v[1], v[6]), dm1 = sub(v[1], v[6]);
+        const C sp2 = add(v[2], v[5]), dm2 = sub(v[2], v[5]);
+        const C sp3 = add(v[3], v[4]), dm3 = sub(v[3], v[4]);
         // q = 1, 2, 3: cos(2 pi r q / 7) and sin(2 pi r q / 7) for r = 1, 2, 3
-        const cd t1 = add(v[0], add(add(scl(sp1, c1), scl(sp2, c2)), scl(sp3, c3)));
-        const cd t2 = add(v[0], add(add(scl(sp1, c2), scl(sp2, c3)), scl(sp3, c1)));
-        const cd t3 = add(v[0], add(add(scl(sp1, c3), scl(sp2, c1)), scl(sp3, c2)));
-        const cd u1 = muli<S>(add(add(scl(dm1, s1), scl(dm2, s2)), scl(dm3, s3)));
-        const cd u2 = muli<S>(sub(sub(scl(dm1, s2), scl(dm2, s3)), scl(dm3, s1)));
-        const cd u3 = muli<S>(add(sub(scl(dm1, s3), scl(dm2, s1)), scl(dm3, s2)));
+        const C t1 = add(v[0], add(add(scl(sp1, c1), scl(sp2, c2)), scl(sp3, c3)));
+        const C t2 = add(v[0], add(add(scl(sp1, c2), scl(sp2, c3)), scl(sp3, c1)));
+        const C t3 = add(v[0], add(add(scl(sp1, c3), scl(sp2, c1)), scl(sp3, c2)));
+        const C u1 = muli<S>(add(add(scl(dm1, s1), scl(dm2, s2)), scl(dm3, s3)));
+        const C u2 = muli<S>(sub(sub(scl(dm1, s2), scl(dm2, s3)), scl(dm3, s1)));
+        const C u3 = muli<S>(add(sub(scl(dm1, s3), scl(dm2, s1)), scl(dm3, s2)));
         v[0] = add(v[0], add(add(sp1, sp2), sp3));
         v[1] = add(t1, u1);
         v[6] = sub(t1, u1);
@@ -182,14 +203,14 @@ __device__ __forceinline__ void dft(cd *v)
 }
 
 // ---- one Stockham stage over NL lines in LDS -----------------------------------
-template <int L, int LS, int NL, int T, int R, int NS, bool INV>
-__device__ __forceinline__ void stage(cd *buf, const cd *__restrict__ tw, int twmul)
+template <int L, int LS, int NL, int T, int R, int NS, bool INV, class C>
+__device__ __forceinline__ void stage(C *buf, const C *__restrict__ tw, int twmul)
 {
     constexpr int LR = L / R;
     constexpr int NB = NL * LR;
     constexpr int PER = (NB + T - 1) / T;
     constexpr int TWS = L / (NS * R);
-    cd v[PER][R];
+    C v[PER][R];
     const int tid = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -198,14 +219,14 @@ __device__ __forceinline__ void stage(cd *buf, const cd *__restrict__ tw, int tw
             const int line = b / LR;
             const int j = b - line * LR;
             const int k = j % NS;
-            const cd *src = buf + line * LS;
+            const C *src = buf + line * LS;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = src[pidx(j + r * LR)];
             if (NS > 1) {
                 // w^r for r = 1..R-1 from one table load: w^2 = w w, w^(2m) = (w^m)^2,
                 // w^(2m+1) = w^(2m) w (at most 3 roundings deep for R <= 8)
-                const cd w1 = tw[opaque(k * TWS * twmul)];
-                cd wp[R];
+                const C w1 = tw[opaque(k * TWS * twmul)];
+                C wp[R];
                 wp[1] = w1;
 #pragma unroll
                 for (int r = 2; r < R; ++r) wp[r] = (r % 2 == 0) ? mul(wp[r / 2], wp[r / 2]) : mul(wp[r - 1], w1);
@@ -222,8 +243,8 @@ __device__ __forceinline__ void stage(cd *buf, const cd *__restrict__ tw, int tw
             const int line = b / LR;
             const int j = b - line * LR;
             const int k = j % NS;
-            dft<R, INV>(v[q]);
-            cd *dst = buf + line * LS;
+            dft<R, INV, C>(v[q]);
+            C *dst = buf + line * LS;
             const int base = (j / NS) * NS * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) dst[pidx(base + r * NS)] = v[q][r];
@@ -232,12 +253,12 @@ __device__ __forceinline__ void stage(cd *buf, const cd *__restrict__ tw, int tw
     __syncthreads();
 }
 
-template <int L, int LS, int NL, int T, bool INV, int S>
-__device__ __forceinline__ void stages_from(cd *buf, const cd *__restrict__ tw, int twmul)
+template <int L, int LS, int NL, int T, bool INV, int S, class C>
+__device__ __forceinline__ void stages_from(C *buf, const C *__restrict__ tw, int twmul)
 {
     if constexpr (S < num_stages(L)) {
-        stage<L, LS, NL, T, stage_radix(L, S), stage_span(L, S), INV>(buf, tw, twmul);
-        stages_from<L, LS, NL, T, INV, S + 1>(buf, tw, twmul);
+        stage<L, LS, NL, T, stage_radix(L, S), stage_span(L, S), INV, C>(buf, tw, twmul);
+        stages_from<L, LS, NL, T, INV, S + 1, C>(buf, tw, twmul);
     }
 }
 
@@ -245,12 +266,12 @@ __device__ __forceinline__ void stages_from(cd *buf, const cd *__restrict__ tw, 
 // buf[l * LS + pidx(i)]; LS >= padded_len(L).  Callers barrier before (data
 // written) -- the first stage reads after the caller's __syncthreads -- and the
 // engine ends with a barrier, so results are visible to every thread on return.
-template <int L, int LS, int NL, int T, bool INV>
-__device__ __forceinline__ void fft_lines(cd *buf, const cd *__restrict__ tw, int twmul)
+template <int L, int LS, int NL, int T, bool INV, class C>
+__device__ __forceinline__ void fft_lines(C *buf, const C *__restrict__ tw, int twmul)
 {
     static_assert(smooth7(L), "line length must factor into 2, 3, 5, 7");
     static_assert(LS >= padded_len(L), "line stride must hold the padded line");
-    stages_from<L, LS, NL, T, INV, 0>(buf, tw, twmul);
+    stages_from<L, LS, NL, T, INV, 0, C>(buf, tw, twmul);
 }
 
 }  // namespace dnsfft
