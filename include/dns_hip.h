@@ -52,6 +52,12 @@ typedef enum {
 } dns_precision;
 
 typedef enum {
+    DNS_PATH_AUTO = 0,       /* fused kernels when built for H = N/2, else the multi-pass path   */
+    DNS_PATH_FUSED = 1,      /* LDS-resident row and column kernels (N <= 4096)                  */
+    DNS_PATH_MULTIPASS = 2   /* batched four-step line FFTs through HBM (large N, e.g. 29400)    */
+} dns_path;
+
+typedef enum {
     DNS_FIELD_U = 0,        /* physical u (UR_full comp 0)                     */
     DNS_FIELD_W = 1,        /* physical w (UR_full comp 1)                     */
     DNS_FIELD_OMEGA = 2,    /* physical vorticity (OM2PHYS, om2phys_cuda.cu:153) */
@@ -67,6 +73,7 @@ typedef struct {
     double cflnum;     /* CFL number (mem_cuda.cu:1755, S.cflnum = CFL)                     */
     int    precision;  /* dns_precision                                                     */
     int    device;     /* HIP device ordinal                                                */
+    int    path;       /* a dns_path value, 0 = auto                                         */
 } dns_config;
 
 /* The scalar part of DnsDeviceState (cuda_dns.h:39-69), in FP64. */
@@ -85,8 +92,10 @@ dns_status dns_create(const dns_config *cfg, dns_handle **out);
 void dns_destroy(dns_handle *h);
 /* Text of the last error on h (or of the last failed dns_create when h is NULL). */
 const char *dns_last_error(const dns_handle *h);
-/* 1 if this library was built with kernels for base grid N, else 0. */
+/* 1 if this library was built with kernels for base grid N (either path), else 0. */
 int dns_supported_n(int N);
+/* 1 if the given dns_path value is built for base grid N, else 0. */
+int dns_supported_path(int N, int path);
 
 /* ---- initial condition ------------------------------------------------ */
 /* Replaces dnsCudaInit = PAO + CALCOM (cuda_dns.h:123; dns_cuda.cu:591-607): sets

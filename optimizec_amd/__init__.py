@@ -18,11 +18,13 @@ import numpy as np
 
 __all__ = ["Solver", "Scalars", "DnsError", "library_path", "load_library", "ic_pao_host",
            "ic_k4exp_host", "supported_n", "FIELD_U", "FIELD_W", "FIELD_OMEGA", "FIELD_PSI",
-           "FIELD_KINETIC", "IC_PAO_KM3", "IC_K4EXP", "PREC_FP64", "PREC_FP32"]
+           "FIELD_KINETIC", "IC_PAO_KM3", "IC_K4EXP", "PREC_FP64", "PREC_FP32",
+           "PATH_AUTO", "PATH_FUSED", "PATH_MULTIPASS"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 IC_PAO_KM3, IC_K4EXP = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
+PATH_AUTO, PATH_FUSED, PATH_MULTIPASS = 0, 1, 2
 FIELD_U, FIELD_W, FIELD_OMEGA, FIELD_PSI, FIELD_KINETIC = 0, 1, 2, 3, 4
 _STATUS = {0: "DNS_OK", 1: "DNS_E_ARG", 2: "DNS_E_UNSUPPORTED", 3: "DNS_E_HIP", 4: "DNS_E_NOMEM",
            5: "DNS_E_STATE", 6: "DNS_E_IO", 7: "DNS_E_FORMAT"}
@@ -36,7 +38,8 @@ class DnsError(RuntimeError):
 
 class Config(ctypes.Structure):
     _fields_ = [("N", ctypes.c_int), ("Re", ctypes.c_double), ("K0", ctypes.c_double),
-                ("cflnum", ctypes.c_double), ("precision", ctypes.c_int), ("device", ctypes.c_int)]
+                ("cflnum", ctypes.c_double), ("precision", ctypes.c_int), ("device", ctypes.c_int),
+                ("path", ctypes.c_int)]
 
 
 class Scalars(ctypes.Structure):
@@ -59,6 +62,7 @@ SIGNATURES = {
     "dns_destroy": (None, [_P]),
     "dns_last_error": (ctypes.c_char_p, [_P]),
     "dns_supported_n": (ctypes.c_int, [ctypes.c_int]),
+    "dns_supported_path": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "dns_init_ic": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_int]),
     "dns_start": (ctypes.c_int, [_P]),
     "dns_step": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -115,6 +119,10 @@ def supported_n(N: int) -> bool:
     return bool(load_library().dns_supported_n(N))
 
 
+def supported_path(N: int, path: int) -> bool:
+    return bool(load_library().dns_supported_path(N, path))
+
+
 def ic_pao_host(N: int, Re: float, K0: float):
     """PAO-k^-3 initial om2 computed on the host: (om2, seed, printed block, visc)."""
     om = np.zeros((N, N // 2), dtype=np.complex128)
@@ -138,10 +146,12 @@ class Solver:
     """One device state: the C handle of dns_create (dnsCudaCreate, cuda_dns.h:108)."""
 
     def __init__(self, N: int, Re: float, K0: float, cflnum: float, device: int = 0,
-                 precision: int = PREC_FP64):
-        """precision: PREC_FP64, or PREC_FP32 (FP32 transforms, FP64 state/update/dt)."""
+                 precision: int = PREC_FP64, path: int = PATH_AUTO):
+        """precision: PREC_FP64, or PREC_FP32 (FP32 transforms, FP64 state/update/dt).
+        path: PATH_AUTO (fused kernels where built, else multi-pass), PATH_FUSED or
+        PATH_MULTIPASS (batched four-step line FFTs through HBM)."""
         L = load_library()
-        cfg = Config(N, float(Re), float(K0), float(cflnum), int(precision), int(device))
+        cfg = Config(N, float(Re), float(K0), float(cflnum), int(precision), int(device), int(path))
         h = _P()
         st = L.dns_create(ctypes.byref(cfg), ctypes.byref(h))
         if st:
@@ -256,14 +266,15 @@ class Solver:
         self._check(load_library().dns_reset_kernel_timing(self._h))
 
     def kernel_timing(self) -> dict:
-        n = ctypes.c_int(8)
-        names = (ctypes.c_char_p * 8)()
-        launches = (ctypes.c_int64 * 8)()
-        ms = (ctypes.c_double * 8)()
-        bytes_ = (ctypes.c_double * 8)()
+        cap = 32
+        n = ctypes.c_int(cap)
+        names = (ctypes.c_char_p * cap)()
+        launches = (ctypes.c_int64 * cap)()
+        ms = (ctypes.c_double * cap)()
+        bytes_ = (ctypes.c_double * cap)()
         self._check(load_library().dns_kernel_timing(self._h, names, launches, ms, bytes_, ctypes.byref(n)))
         return {names[i].decode(): {"launches": launches[i], "total_ms": ms[i], "bytes_per_launch": bytes_[i]}
-                for i in range(n.value)}
+                for i in range(min(cap, n.value))}
 
     def device_bytes(self) -> int:
         return int(load_library().dns_device_bytes(self._h))

@@ -1,8 +1,9 @@
 """Builds the gfx950 solver library ``optimizec_amd/libdns_hip.so`` and the host driver
 ``optimizec_amd/mem_hip`` in-tree.
 
-* ``csrc/dns_kernels.hip`` -- device code, hipcc --offload-arch=gfx950 (packed FP32 ops
-  disabled for every HIP translation unit).
+* ``csrc/dns_kernels.hip`` (fused kernels), ``csrc/dns_big.hip`` (multi-pass path) --
+  device code, hipcc --offload-arch=gfx950 (packed FP32 ops disabled for every HIP
+  translation unit); the two compile in parallel.
 * ``csrc/dns_api.cpp``, ``csrc/ic_host.cpp`` -- host code, g++ with -ffp-contract=off so
   the FP32 parts of the reference initial condition round exactly like the reference.
 * ``driver/mem_hip.cpp`` -- the command-line driver, linked against the library.
@@ -42,9 +43,16 @@ def _run(cmd: list[str]) -> None:
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    k_obj = os.path.join(BUILD, "dns_kernels.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "dns_kernels.hip"), "-o", k_obj])
-    objs.append(k_obj)
+    procs = []
+    for src in ("dns_kernels.hip", "dns_big.hip"):
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
+        objs.append(obj)
+    for cmd, p in procs:
+        if p.wait() != 0:
+            raise subprocess.CalledProcessError(p.returncode, cmd)
     for src in ("dns_api.cpp", "ic_host.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/dns_hip.h"
+#include "dns_big.h"
 #include "dns_kernels.h"
 
 using dnsfft::cd;
@@ -34,8 +35,25 @@ struct TimedLaunch {
     hipEvent_t a, b;
 };
 
-enum { KT_ROW = 0, KT_COL = 1, KT_ROW_CFL = 2, KT_COUNT = 3 };
-const char *const kKernelNames[KT_COUNT] = {"k_row", "k_col", "k_row_cfl"};
+// timed kernels (fused path) and timed phases (multi-pass path, each one or more launches)
+enum {
+    KT_ROW = 0,
+    KT_COL,
+    KT_ROW_CFL,
+    KT_BIG_SPEC,
+    KT_BIG_ZINV,
+    KT_BIG_PACK,
+    KT_BIG_XINV,
+    KT_BIG_PROD,
+    KT_BIG_XFWD,
+    KT_BIG_UNPACK,
+    KT_BIG_ZFWD,
+    KT_BIG_CN,
+    KT_COUNT
+};
+const char *const kKernelNames[KT_COUNT] = {"k_row",    "k_col",     "k_row_cfl", "big_spec",
+                                            "big_zinv", "big_pack",  "big_xinv",  "big_prod",
+                                            "big_xfwd", "big_unpack", "big_zfwd", "big_cn"};
 
 }  // namespace
 
@@ -43,6 +61,11 @@ struct dns_handle {
     int N = 0, M = 0, H = 0, NK = 0;
     int device = 0;
     int fp32 = 0;               // 1: V, G and the transforms in FP32 (state stays FP64)
+    bool big = false;           // multi-pass path (dns_big.hip) instead of the fused kernels
+    void *Zb = nullptr;         // multi-pass: 4 z-line fields [f][kx][z] (transform precision)
+    void *Rb = nullptr;         // multi-pass: 2 row planes [p][z][x]
+    void *Sb = nullptr;         // multi-pass: line-FFT scratch; also diagnostics plane and staging
+    size_t zb_bytes = 0, rb_bytes = 0, sb_bytes = 0;
     dns_scalars sc{};
     hipStream_t stream = nullptr;
     cd *om = nullptr, *fn = nullptr, *tw = nullptr;
@@ -61,8 +84,8 @@ struct dns_handle {
     bool timing = false;
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> event_pool;
-    int64_t launches[KT_COUNT] = {0, 0, 0};
-    double total_ms[KT_COUNT] = {0, 0, 0};
+    int64_t launches[KT_COUNT] = {};
+    double total_ms[KT_COUNT] = {};
 };
 
 namespace {
@@ -131,9 +154,8 @@ void drain_timing(dns_handle *h)
     h->pending.clear();
 }
 
-dns_status launch(dns_handle *h, int timed_kernel, LaunchReq &r)
+TimedLaunch timing_begin(dns_handle *h, int timed_kernel)
 {
-    r.stream = h->stream;
     TimedLaunch tl{timed_kernel, nullptr, nullptr};
     if (h->timing && timed_kernel >= 0) {
         if (h->pending.size() > 4096) drain_timing(h);
@@ -141,12 +163,152 @@ dns_status launch(dns_handle *h, int timed_kernel, LaunchReq &r)
         tl.b = take_event(h);
         if (tl.a && tl.b) (void)hipEventRecord(tl.a, h->stream);
     }
-    hipError_t e = dns_kernels_launch(h->H, r);
-    if (e != hipSuccess) return fail(h, DNS_E_HIP, "kernel launch (%d) failed: %s", r.what, hipGetErrorString(e));
+    return tl;
+}
+
+void timing_end(dns_handle *h, const TimedLaunch &tl)
+{
     if (tl.a && tl.b) {
         (void)hipEventRecord(tl.b, h->stream);
         h->pending.push_back(tl);
     }
+}
+
+dns_status launch(dns_handle *h, int timed_kernel, LaunchReq &r)
+{
+    r.stream = h->stream;
+    const TimedLaunch tl = timing_begin(h, timed_kernel);
+    hipError_t e = dns_kernels_launch(h->H, r);
+    if (e != hipSuccess) return fail(h, DNS_E_HIP, "kernel launch (%d) failed: %s", r.what, hipGetErrorString(e));
+    timing_end(h, tl);
+    return DNS_OK;
+}
+
+// ---- multi-pass path ------------------------------------------------------------
+dns_status big_launch(dns_handle *h, BigReq &r)
+{
+    r.stream = h->stream;
+    r.H = h->H;
+    hipError_t e = dns_big_launch(r);
+    if (e != hipSuccess)
+        return fail(h, DNS_E_HIP, "multi-pass kernel launch (%d) failed: %s", r.what, hipGetErrorString(e));
+    return DNS_OK;
+}
+
+// `lines` line FFTs of length M from src to dst (may be equal), in chunks that fit the
+// scratch buffer; fp64 selects FP64 elements whatever the handle's precision.
+dns_status big_fft(dns_handle *h, bool fp64, bool inverse, void *src, void *dst, size_t lines)
+{
+    const size_t elem = (fp64 || !h->fp32) ? sizeof(cd) : sizeof(dnsfft::cf);
+    const size_t line_bytes = elem * (size_t)h->M;
+    size_t chunk = h->sb_bytes / line_bytes;
+    if (chunk > 65535) chunk = 65535;
+    if (chunk == 0) return fail(h, DNS_E_STATE, "line-FFT scratch holds no line");
+    for (size_t l0 = 0; l0 < lines; l0 += chunk) {
+        const size_t n = std::min(chunk, lines - l0);
+        BigReq r{};
+        r.what = inverse ? BIG_FFT_INV : BIG_FFT_FWD;
+        r.fp32 = fp64 ? 0 : h->fp32;
+        r.tw = r.fp32 ? h->twT : h->tw;
+        r.a = static_cast<char *>(src) + l0 * line_bytes;
+        r.b = static_cast<char *>(dst) + l0 * line_bytes;
+        r.scratch = h->Sb;
+        r.lines = n;
+        dns_status st = big_launch(h, r);
+        if (st != DNS_OK) return st;
+    }
+    return DNS_OK;
+}
+
+// physical rows of nf spectra of the current om: fields paired as a + i b into the row
+// planes Rb (pair p at plane p); timed phases only on the step's own call (timed).
+dns_status big_inverse(dns_handle *h, bool fp64, int nf, const int *fields, bool timed)
+{
+    const int fp32 = fp64 ? 0 : h->fp32;
+    const size_t elem = fp32 ? sizeof(dnsfft::cf) : sizeof(cd);
+    const size_t nmid = (size_t)h->M * h->H, nrow = (size_t)h->M * h->M;
+    if (nf < 1 || nf > 4 || elem * nmid * nf > h->zb_bytes || elem * nrow * ((nf + 1) / 2) > h->rb_bytes)
+        return fail(h, DNS_E_STATE, "multi-pass buffers too small for %d fields", nf);
+    TimedLaunch tl = timing_begin(h, timed ? KT_BIG_SPEC : -1);
+    BigReq r{};
+    r.what = BIG_SPEC;
+    r.fp32 = fp32;
+    r.om = h->om;
+    r.a = h->Zb;
+    r.spec.nfields = nf;
+    for (int i = 0; i < nf; ++i) r.spec.fields[i] = fields[i];
+    dns_status st = big_launch(h, r);
+    if (st != DNS_OK) return st;
+    timing_end(h, tl);
+    tl = timing_begin(h, timed ? KT_BIG_ZINV : -1);
+    st = big_fft(h, fp64, true, h->Zb, h->Zb, (size_t)nf * h->H);
+    if (st != DNS_OK) return st;
+    timing_end(h, tl);
+    tl = timing_begin(h, timed ? KT_BIG_PACK : -1);
+    for (int p = 0; p < (nf + 1) / 2; ++p) {
+        BigReq k{};
+        k.what = BIG_PACK;
+        k.fp32 = fp32;
+        k.a = static_cast<char *>(h->Zb) + (size_t)(2 * p) * nmid * elem;
+        k.a2 = (2 * p + 1 < nf) ? static_cast<char *>(h->Zb) + (size_t)(2 * p + 1) * nmid * elem : nullptr;
+        k.b = static_cast<char *>(h->Rb) + (size_t)p * nrow * elem;
+        st = big_launch(h, k);
+        if (st != DNS_OK) return st;
+    }
+    timing_end(h, tl);
+    tl = timing_begin(h, timed ? KT_BIG_XINV : -1);
+    st = big_fft(h, fp64, true, h->Rb, h->Rb, (size_t)((nf + 1) / 2) * h->M);
+    if (st != DNS_OK) return st;
+    timing_end(h, tl);
+    return DNS_OK;
+}
+
+// one time step of the multi-pass path (same arithmetic as the fused row + column pass)
+dns_status big_step(dns_handle *h, double dt)
+{
+    static const int f4[4] = {DNS_F_U, DNS_F_W, DNS_F_OX, DNS_F_OZ};
+    const size_t elem = h->fp32 ? sizeof(dnsfft::cf) : sizeof(cd);
+    const size_t nrow = (size_t)h->M * h->M;
+    dns_status st = big_inverse(h, false, 4, f4, true);
+    if (st != DNS_OK) return st;
+    // row-pair products into the z-line buffer (free after the pack)
+    TimedLaunch tl = timing_begin(h, KT_BIG_PROD);
+    BigReq p{};
+    p.what = BIG_PROD;
+    p.fp32 = h->fp32;
+    p.a = h->Rb;
+    p.a2 = static_cast<char *>(h->Rb) + nrow * elem;
+    p.b = h->Zb;
+    if ((st = big_launch(h, p)) != DNS_OK) return st;
+    timing_end(h, tl);
+    tl = timing_begin(h, KT_BIG_XFWD);
+    if ((st = big_fft(h, false, false, h->Zb, h->Zb, (size_t)h->M / 2)) != DNS_OK) return st;
+    timing_end(h, tl);
+    // kx < H of every row, as z-lines into the row buffer (free after the product)
+    tl = timing_begin(h, KT_BIG_UNPACK);
+    BigReq u{};
+    u.what = BIG_UNPACK;
+    u.fp32 = h->fp32;
+    u.a = h->Zb;
+    u.b = h->Rb;
+    if ((st = big_launch(h, u)) != DNS_OK) return st;
+    timing_end(h, tl);
+    tl = timing_begin(h, KT_BIG_ZFWD);
+    if ((st = big_fft(h, false, false, h->Rb, h->Rb, (size_t)h->H)) != DNS_OK) return st;
+    timing_end(h, tl);
+    tl = timing_begin(h, KT_BIG_CN);
+    BigReq c{};
+    c.what = BIG_CN;
+    c.fp32 = h->fp32;
+    c.om = h->om;
+    c.fn = h->fn;
+    c.a = h->Rb;
+    c.cn.divxz = 1.0 / (double(h->M) * double(h->M));
+    c.cn.VT = 0.5 * h->sc.visc * dt;
+    c.cn.c2 = 0.5 * dt * (2.0 + h->sc.cnm1);
+    c.cn.c3 = -0.5 * dt * h->sc.cnm1;
+    if ((st = big_launch(h, c)) != DNS_OK) return st;
+    timing_end(h, tl);
     return DNS_OK;
 }
 
@@ -165,6 +327,10 @@ ColArgs inverse_args(dns_handle *h, int nf, const int *fields, size_t vstride)
 dns_status refresh_v(dns_handle *h)
 {
     static const int f4[4] = {DNS_F_U, DNS_F_W, DNS_F_OX, DNS_F_OZ};
+    if (h->big) {  // the multi-pass step rebuilds every intermediate from om
+        h->v_valid = true;
+        return DNS_OK;
+    }
     LaunchReq r{};
     r.what = LAUNCH_COL;
     r.fp32 = h->fp32;
@@ -216,6 +382,17 @@ dns_status compute_field(dns_handle *h, int field)
             break;
         default: return fail(h, DNS_E_ARG, "unknown field %d", field);
     }
+    if (h->big) {
+        st = big_inverse(h, true, nf, fields, false);
+        if (st != DNS_OK) return st;
+        BigReq p{};
+        p.what = BIG_PHYS;
+        p.fp32 = 0;
+        p.a = h->Rb;
+        p.out0 = h->phys;
+        p.out_mode = (field == DNS_FIELD_KINETIC) ? 1 : 0;
+        return big_launch(h, p);
+    }
     const size_t vs = (size_t)h->M * h->H;
     LaunchReq r{};
     r.what = LAUNCH_COL;
@@ -258,7 +435,20 @@ dns_status compute_cflm(dns_handle *h, double *cflm)
     r.cflrow = h->cflrow;
     r.DX = 2.0 * PI / double(h->N);
     r.DZ = 2.0 * PI / double(h->N);
-    if (!h->fp32) {
+    if (h->big) {
+        static const int f2[2] = {DNS_F_U, DNS_F_W};
+        dns_status st = big_inverse(h, true, 2, f2, false);
+        if (st != DNS_OK) return st;
+        BigReq c{};
+        c.what = BIG_CFL;
+        c.fp32 = 0;
+        c.a = h->Rb;
+        c.cflrow = h->cflrow;
+        c.DX = r.DX;
+        c.DZ = r.DZ;
+        st = big_launch(h, c);
+        if (st != DNS_OK) return st;
+    } else if (!h->fp32) {
         if (!h->v_valid) {
             dns_status st = refresh_v(h);
             if (st != DNS_OK) return st;
@@ -281,8 +471,10 @@ dns_status compute_cflm(dns_handle *h, double *cflm)
         if (st != DNS_OK) return st;
         r.V = h->Vd;
     }
-    dns_status st = launch(h, KT_ROW_CFL, r);
-    if (st != DNS_OK) return st;
+    if (!h->big) {
+        dns_status st = launch(h, KT_ROW_CFL, r);
+        if (st != DNS_OK) return st;
+    }
     std::vector<double> rows(h->M);
     HIPCHK(h, hipMemcpyAsync(rows.data(), h->cflrow, sizeof(double) * h->M, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -332,10 +524,19 @@ void reset_scalars(dns_handle *h)
 // =====================================================================================
 // life cycle
 // =====================================================================================
-extern "C" int dns_supported_n(int N)
+extern "C" int dns_supported_path(int N, int path)
 {
-    return (N >= 16 && N % 4 == 0 && dns_kernels_supported(N / 2)) ? 1 : 0;
+    if (N < 16 || N % 4 != 0) return 0;
+    const bool fused = dns_kernels_supported(N / 2) != 0, big = dns_big_supported_m(3 * N / 2) != 0;
+    switch (path) {
+        case DNS_PATH_AUTO: return (fused || big) ? 1 : 0;
+        case DNS_PATH_FUSED: return fused ? 1 : 0;
+        case DNS_PATH_MULTIPASS: return big ? 1 : 0;
+        default: return 0;
+    }
 }
+
+extern "C" int dns_supported_n(int N) { return dns_supported_path(N, DNS_PATH_AUTO); }
 
 extern "C" const char *dns_last_error(const dns_handle *h)
 {
@@ -347,8 +548,11 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     if (!cfg || !out) return fail(nullptr, DNS_E_ARG, "dns_create: null argument");
     *out = nullptr;
     if (cfg->N < 4 || cfg->N % 4 != 0) return fail(nullptr, DNS_E_ARG, "N=%d must be a positive multiple of 4", cfg->N);
-    if (!dns_supported_n(cfg->N))
-        return fail(nullptr, DNS_E_UNSUPPORTED, "N=%d: no kernels built for H=N/2=%d", cfg->N, cfg->N / 2);
+    if (cfg->path < DNS_PATH_AUTO || cfg->path > DNS_PATH_MULTIPASS)
+        return fail(nullptr, DNS_E_ARG, "unknown path %d", cfg->path);
+    if (!dns_supported_path(cfg->N, cfg->path))
+        return fail(nullptr, DNS_E_UNSUPPORTED, "N=%d: no kernels built for H=N/2=%d, M=3N/2=%d on path %d", cfg->N,
+                    cfg->N / 2, 3 * cfg->N / 2, cfg->path);
     if (cfg->precision != DNS_PREC_FP64 && cfg->precision != DNS_PREC_FP32)
         return fail(nullptr, DNS_E_UNSUPPORTED, "precision %d is not built", cfg->precision);
     if (cfg->precision == DNS_PREC_FP32 && cfg->N < 128)
@@ -367,6 +571,7 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     h->NK = 3 * cfg->N / 4 + 1;
     h->device = cfg->device;
     h->fp32 = cfg->precision == DNS_PREC_FP32 ? 1 : 0;
+    h->big = cfg->path == DNS_PATH_MULTIPASS || (cfg->path == DNS_PATH_AUTO && !dns_kernels_supported(h->H));
     h->sc.N = h->N;
     h->sc.M = h->M;
     h->sc.H = h->H;
@@ -393,8 +598,23 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     if ((st = dev_alloc(h, (void **)&h->om, sizeof(cd) * nspec, "om2")) != DNS_OK) return bail(st);
     if ((st = dev_alloc(h, (void **)&h->fn, sizeof(cd) * nspec, "fnm1")) != DNS_OK) return bail(st);
     const size_t elem = h->fp32 ? sizeof(dnsfft::cf) : sizeof(cd);
-    if ((st = dev_alloc(h, &h->V, elem * 4 * nmid, "z-inverse fields")) != DNS_OK) return bail(st);
-    if ((st = dev_alloc(h, &h->G, elem * nmid, "x-forward product")) != DNS_OK) return bail(st);
+    if (h->big) {
+        // z-lines: 4 fields; rows: 2 planes; scratch: one plane (the FFTs run in chunks
+        // of at most M lines).  Diagnostics run in FP64 inside the same bytes: 2 fields,
+        // 1 plane, and the FP64 physical plane / host staging in the scratch.
+        const size_t nrow = (size_t)h->M * h->M;
+        h->zb_bytes = elem * 4 * nmid;
+        h->rb_bytes = elem * 2 * nrow;
+        h->sb_bytes = std::max(elem * nrow, sizeof(double) * nrow);
+        if ((st = dev_alloc(h, &h->Zb, h->zb_bytes, "z-line fields")) != DNS_OK) return bail(st);
+        if ((st = dev_alloc(h, &h->Rb, h->rb_bytes, "row planes")) != DNS_OK) return bail(st);
+        if ((st = dev_alloc(h, &h->Sb, h->sb_bytes, "line-FFT scratch")) != DNS_OK) return bail(st);
+        h->Vd = static_cast<cd *>(h->Sb);
+        h->phys = static_cast<double *>(h->Sb);
+    } else {
+        if ((st = dev_alloc(h, &h->V, elem * 4 * nmid, "z-inverse fields")) != DNS_OK) return bail(st);
+        if ((st = dev_alloc(h, &h->G, elem * nmid, "x-forward product")) != DNS_OK) return bail(st);
+    }
     if ((st = dev_alloc(h, (void **)&h->tw, sizeof(cd) * h->M, "twiddles")) != DNS_OK) return bail(st);
     if ((st = dev_alloc(h, (void **)&h->cflrow, sizeof(double) * h->M, "cfl rows")) != DNS_OK) return bail(st);
     if ((st = dev_alloc(h, (void **)&h->partEZ, sizeof(double) * 2 * h->H, "energy partials")) != DNS_OK)
@@ -439,7 +659,12 @@ extern "C" void dns_destroy(dns_handle *h)
     }
     for (hipEvent_t ev : h->event_pool) (void)hipEventDestroy(ev);
     if (h->twT == h->tw) h->twT = nullptr;
-    void *ptrs[] = {h->om, h->fn, h->V, h->G, h->tw, h->twT, h->Vd, h->phys, h->cflrow, h->partEZ, h->red, h->redu};
+    if (h->big) {  // views of the scratch buffer
+        h->Vd = nullptr;
+        h->phys = nullptr;
+    }
+    void *ptrs[] = {h->om, h->fn, h->V,     h->G,      h->tw,  h->twT, h->Vd, h->phys,
+                    h->cflrow, h->partEZ, h->red, h->redu, h->Zb, h->Rb, h->Sb};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -498,6 +723,13 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
     for (int s = 0; s < nsteps; ++s) {
         h->sc.it += 1;
         const double dt = h->sc.dt;
+        if (h->big) {
+            dns_status st = big_step(h, dt);
+            if (st != DNS_OK) return st;
+            h->sc.t += dt;
+            h->sc.cnm1 = h->sc.cn;
+            continue;
+        }
         LaunchReq r{};
         r.what = LAUNCH_ROW;
         r.fp32 = h->fp32;
@@ -875,8 +1107,17 @@ extern "C" dns_status dns_kernel_timing(dns_handle *h, const char **names, int64
     //   k_row     reads 4 fields M x H, writes 1: 5 * 0.75 N^2 * c           (60 N^2 | 30 N^2)
     //   k_col     reads G + om, fn, writes om, fn + V: 3.75 c N^2 + 32 N^2   (92 N^2 | 62 N^2)
     //   k_row_cfl reads u, w in FP64: 24 N^2
+    // multi-pass phases (per call; every line-FFT pass reads and writes its batch once):
+    //   big_spec   om in (8 N^2), 4 z-line fields out (3 c N^2)
+    //   big_zinv   2 passes over 4 fields: 12 c N^2      big_pack  3 c in, 4.5 c out
+    //   big_xinv   2 passes over 2 planes: 18 c N^2      big_prod  4.5 c in, 1.125 c out
+    //   big_xfwd   2 passes over 1/2 plane: 4.5 c N^2    big_unpack 1.125 c in, 0.75 c out
+    //   big_zfwd   2 passes over 1 field: 3 c N^2        big_cn    0.75 c in + 32 N^2 state
     const double c = h->fp32 ? 8.0 : 16.0;
-    const double bytes[KT_COUNT] = {3.75 * c * N2, (3.75 * c + 32.0) * N2, 24.0 * N2};
+    const double bytes[KT_COUNT] = {3.75 * c * N2, (3.75 * c + 32.0) * N2, 24.0 * N2,
+                                    (8.0 + 3.0 * c) * N2, 12.0 * c * N2, 7.5 * c * N2,
+                                    18.0 * c * N2, 5.625 * c * N2, 4.5 * c * N2,
+                                    1.875 * c * N2, 3.0 * c * N2, (0.75 * c + 32.0) * N2};
     const int cnt = std::min(*n, (int)KT_COUNT);
     for (int i = 0; i < cnt; ++i) {
         if (names) names[i] = kKernelNames[i];

@@ -556,7 +556,7 @@ int main(int argc, char **argv)
     r.K0 = K0;
     r.Re = Re;
     r.CFL = CFL;
-    dns_config cfg{N, Re, K0, CFL, DNS_PREC_FP64, 0};
+    dns_config cfg{N, Re, K0, CFL, DNS_PREC_FP64, 0, DNS_PATH_AUTO};
     dns_status st = dns_create(&cfg, &r.h);
     if (st != DNS_OK) {
         std::fprintf(stderr, "dns_create: %s\n", dns_last_error(nullptr));

@@ -30,10 +30,20 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_supported_sizes():
-    for N in (16, 32, 64, 128, 256, 512, 768, 1024, 2048, 3072, 4096):
+    for N in (16, 32, 64, 128, 256, 512, 768, 1024, 2048, 3072, 4096, 8192, 16384, 29400):
         assert oa.supported_n(N), N
-    for N in (10, 30, 100, 8192):
+    for N in (10, 30, 100, 6000, 29404):
         assert not oa.supported_n(N), N
+    # fused kernels up to N = 4096; the multi-pass path for the large grids and for
+    # the sizes the parity tests force through it
+    for N in (4096, 3072, 16):
+        assert oa.supported_path(N, oa.PATH_FUSED), N
+    for N in (8192, 16384, 29400):
+        assert not oa.supported_path(N, oa.PATH_FUSED), N
+    for N in (64, 128, 256, 512, 1024, 4096, 8192, 16384, 29400):
+        assert oa.supported_path(N, oa.PATH_MULTIPASS), N
+    assert not oa.supported_path(2048, oa.PATH_MULTIPASS)
+    assert not oa.supported_path(512, 7)
 
 
 def test_pao_host_ic_matches_oracle(oracle_mod):
@@ -66,11 +76,15 @@ def test_create_fails_loudly_without_device():
 
 def test_bad_arguments_are_rejected():
     lib = oa.load_library()
-    cfg = oa.Config(30, 1e3, 4.0, 0.1, 0, 0)
+    cfg = oa.Config(30, 1e3, 4.0, 0.1, 0, 0, 0)
     h = ctypes.c_void_p()
     assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 1  # DNS_E_ARG: N % 4
     assert b"multiple of 4" in lib.dns_last_error(None)
-    cfg = oa.Config(8192 * 2, 1e3, 4.0, 0.1, 0, 0)
+    cfg = oa.Config(6000, 1e3, 4.0, 0.1, 0, 0, 0)
     assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 2  # DNS_E_UNSUPPORTED
+    cfg = oa.Config(8192, 1e3, 4.0, 0.1, 0, 0, oa.PATH_FUSED)
+    assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 2  # no fused kernels for N=8192
+    cfg = oa.Config(512, 1e3, 4.0, 0.1, 0, 0, 9)
+    assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 1  # unknown path
     with pytest.raises(oa.DnsError):
         oa.ic_k4exp_host(30, 4.0, 0)
