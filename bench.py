@@ -1,10 +1,16 @@
-"""Benchmark of the FP64 time step at N=4096 (6144^2 padded grid) on one MI355X.
+"""Benchmark of the FP64 time step at N=4096 (6144^2 padded grid) on one MI355X, with the
+N=29400 (44100^2 padded) step beside it.
 
 One "step" is one pass of the mem_cuda loop body (mem_cuda.cu:1848-1857): the row
 kernel (x-inverse of u, w, grad omega -> u.grad(omega) -> x-forward) and the column
 kernel (z-forward -> CN/AB2 update -> z-inverse of the new omega), with NEXTDT on the
 reference cadence (it % 100 == 0).  The state is resident in HBM before the timed
 region; the initial condition is the seeded k^4 exp(-2 k^2/K0^2) spectrum.
+
+The N=29400 leg (Re=1e7, K0=4, CFL=0.1, 41 steps, NEXTDT at it=1, 20, 40, 41; steps
+2-41 timed) runs the multi-pass path with the whole FP64 state resident on the GPU, in
+FP64 and in mixed precision, and reports the relative L2 vorticity difference between
+the two after step 41 under "n29400".
 
 Prints one JSON line (see the keys below).  ``--gpus`` above 1 is refused: this build
 runs on one GPU.
@@ -23,6 +29,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 UPDATE = 100
 
 WORKLOAD = {"N": 4096, "Re": 1.0e5, "K0": 4.0, "CFL": 0.1, "seed": 0}
+BIG = {"N": 29400, "Re": 1.0e7, "K0": 4.0, "CFL": 0.1, "seed": 0, "steps": 41, "update": 20}
 
 
 def _traffic_from_profile(kernel: str, N: int):
@@ -33,7 +40,7 @@ def _traffic_from_profile(kernel: str, N: int):
         return None
     with open(path, encoding="utf-8") as f:
         d = json.load(f)
-    k = d.get("kernels", {}).get(kernel)
+    k = d.get("by_timer_name", {}).get(kernel)
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
@@ -61,6 +68,66 @@ def cpu_baseline(N: int, seconds: float = 15.0):
             "sample": f"N={N} FP64 advective-form oracle, {len(times)} steps after init, median"}
 
 
+def big_leg(N: int) -> dict:
+    """The large-grid configuration: FP64 and mixed precision, steps 2..41 timed."""
+    import numpy as np
+
+    import optimizec_amd as oa
+
+    def run(prec):
+        s = oa.Solver(N, BIG["Re"], BIG["K0"], BIG["CFL"], precision=prec)
+        s.init_ic(oa.IC_K4EXP, BIG["seed"])
+        s.start()
+        steps, update = BIG["steps"], BIG["update"]
+
+        def one():
+            s.step(1)
+            it = s.scalars().it
+            if it == 1 or it % update == 0 or it == steps:
+                s.next_dt()
+
+        one()
+        s.synchronize()
+        s.set_kernel_timing(True)
+        s.reset_kernel_timing()
+        t0 = time.perf_counter()
+        for _ in range(steps - 1):
+            one()
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        kt = s.kernel_timing()
+        s.set_kernel_timing(False)
+        E, Z = s.energy_enstrophy()
+        om = s.om2()
+        dev = s.device_bytes()
+        s.close()
+        phases = {k: round(d["total_ms"] / d["launches"], 3) for k, d in kt.items() if d["launches"]}
+        gbps = {k: round(d["bytes_per_launch"] / (d["total_ms"] / d["launches"] * 1e-3) / 1e9, 1)
+                for k, d in kt.items() if d["launches"]}
+        return 1e3 * wall / (steps - 1), phases, gbps, E, Z, om, dev
+
+    ms64, ph64, gb64, E, Z, om64, dev64 = run(oa.PREC_FP64)
+    ms32, ph32, _, E32, Z32, om32, dev32 = run(oa.PREC_FP32)
+    err = float(np.linalg.norm(om32 - om64) / np.linalg.norm(om64))
+    del om32, om64
+    return {
+        "workload": f"2D vorticity DNS, N={N} ({3 * N // 2}^2 padded), Re=1e7, K0=4, CFL=0.1, "
+                    f"{BIG['steps']} steps, NEXTDT at it=1, {BIG['update']}, 40, 41; steps 2-41 timed",
+        "path": "multi-pass (batched four-step line FFTs through HBM)",
+        "fp64_ms_per_step": round(ms64, 3),
+        "fp64_step_hbm_GBps": round(160.0 * N * N / (ms64 * 1e-3) / 1e9, 1),
+        "fp64_phase_ms": ph64,
+        "fp64_phase_GBps": gb64,
+        "fp64_device_GB": round(dev64 / 1e9, 1),
+        "mixed_ms_per_step": round(ms32, 3),
+        "mixed_speedup_vs_fp64": round(ms64 / ms32, 3),
+        "mixed_phase_ms": ph32,
+        "mixed_device_GB": round(dev32 / 1e9, 1),
+        "mixed_rel_l2_vorticity_err_vs_fp64_step41": err,
+        "final_fp64": {"E": E, "Z": Z}, "final_mixed": {"E": E32, "Z": Z32},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +136,7 @@ def main() -> int:
     ap.add_argument("--N", type=int, default=WORKLOAD["N"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--big-N", type=int, default=BIG["N"], help="large-grid leg (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
@@ -144,7 +212,7 @@ def main() -> int:
         avg_ms = d["total_ms"] / d["launches"]
         per_kernel[name] = {"avg_ms": round(avg_ms, 5), "launches": d["launches"],
                             "algorithmic_GBps": round(d["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9, 1)}
-    dom = max(("k_row", "k_col"), key=lambda k: kt[k]["total_ms"])
+    dom = max(kt, key=lambda k: kt[k]["total_ms"])
     dom_avg_ms = kt[dom]["total_ms"] / max(1, kt[dom]["launches"])
     achieved = kt[dom]["bytes_per_launch"] / (dom_avg_ms * 1e-3) / 1e9
     step_bytes = 152.0 * N * N  # 2.5 c_s + 7.5 c_i with the om2 re-read fused away
@@ -175,6 +243,8 @@ def main() -> int:
         "device_bytes": dev_bytes,
         "final": {"it": sc.it, "t": sc.t, "dt": sc.dt, "E": E, "Z": Z, "finite": finite},
     }
+    if args.big_N > 0:
+        out["n29400" if args.big_N == 29400 else f"n{args.big_N}"] = big_leg(args.big_N)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N)
     print(json.dumps(out), flush=True)
