@@ -52,11 +52,22 @@ class Scalars(ctypes.Structure):
     ]
 
 
+class EnsMember(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("Re", ctypes.c_double), ("K0", ctypes.c_double),
+                ("cflnum", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class EnsConfig(ctypes.Structure):
+    _fields_ = [("nmembers", ctypes.c_int), ("members", ctypes.POINTER(EnsMember)),
+                ("pair_every", ctypes.c_int), ("filter_ratio", ctypes.c_int), ("max_pairs", ctypes.c_int),
+                ("use_graph", ctypes.c_int), ("device", ctypes.c_int)]
+
+
 _lib = None
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
 
-# every symbol include/dns_hip.h declares, with its ctypes signature
+# every symbol include/dns_hip.h and include/dns_ensemble.h declare, with its ctypes signature
 SIGNATURES = {
     "dns_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_P)]),
     "dns_destroy": (None, [_P]),
@@ -88,6 +99,23 @@ SIGNATURES = {
     "dns_reset_kernel_timing": (ctypes.c_int, [_P]),
     "dns_synchronize": (ctypes.c_int, [_P]),
     "dns_device_bytes": (ctypes.c_size_t, [_P]),
+    # batched ensemble (include/dns_ensemble.h)
+    "dns_ens_create": (ctypes.c_int, [ctypes.POINTER(EnsConfig), ctypes.POINTER(_P)]),
+    "dns_ens_destroy": (None, [_P]),
+    "dns_ens_last_error": (ctypes.c_char_p, [_P]),
+    "dns_ens_init": (ctypes.c_int, [_P]),
+    "dns_ens_set_om2": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "dns_ens_advance": (ctypes.c_int, [_P, ctypes.c_int]),
+    "dns_ens_synchronize": (ctypes.c_int, [_P]),
+    "dns_ens_get_scalars": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(Scalars)]),
+    "dns_ens_set_scalars": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(Scalars)]),
+    "dns_ens_get_om2": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "dns_ens_get_fnm1": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "dns_ens_pair_info": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int)]),
+    "dns_ens_get_pairs": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_float)]),
+    "dns_ens_device_bytes": (ctypes.c_size_t, [_P]),
 }
 
 
@@ -278,3 +306,90 @@ class Solver:
 
     def device_bytes(self) -> int:
         return int(load_library().dns_device_bytes(self._h))
+
+
+class Ensemble:
+    """Many independent runs advanced together (include/dns_ensemble.h): members given as
+    (N, Re, K0, cflnum, seed) tuples; training pairs every ``pair_every`` steps on the
+    N / ``filter_ratio`` grid, channels (filtered omega, u, w, subgrid term)."""
+
+    def __init__(self, members, pair_every: int = 20, filter_ratio: int = 8, max_pairs: int = 100,
+                 use_graph: bool = True, device: int = 0):
+        L = load_library()
+        self.members = [tuple(m) for m in members]
+        arr = (EnsMember * len(self.members))()
+        for i, (N, Re, K0, cfl, seed) in enumerate(self.members):
+            arr[i] = EnsMember(int(N), float(Re), float(K0), float(cfl), int(seed))
+        self._arr = arr
+        cfg = EnsConfig(len(self.members), arr, int(pair_every), int(filter_ratio), int(max_pairs),
+                        1 if use_graph else 0, int(device))
+        h = _P()
+        st = L.dns_ens_create(ctypes.byref(cfg), ctypes.byref(h))
+        if st:
+            raise DnsError(st, L.dns_ens_last_error(None).decode())
+        self._h = h
+        self.pair_every = pair_every
+
+    def _check(self, st: int) -> None:
+        if st:
+            raise DnsError(st, load_library().dns_ens_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load_library().dns_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self) -> None:
+        self._check(load_library().dns_ens_init(self._h))
+
+    def advance(self, nsteps: int) -> None:
+        self._check(load_library().dns_ens_advance(self._h, nsteps))
+
+    def synchronize(self) -> None:
+        self._check(load_library().dns_ens_synchronize(self._h))
+
+    def scalars(self, m: int) -> Scalars:
+        s = Scalars()
+        self._check(load_library().dns_ens_get_scalars(self._h, m, ctypes.byref(s)))
+        return s
+
+    def set_scalars(self, m: int, s: Scalars) -> None:
+        self._check(load_library().dns_ens_set_scalars(self._h, m, ctypes.byref(s)))
+
+    def om2(self, m: int) -> np.ndarray:
+        N = self.members[m][0]
+        out = np.empty((N, N // 2), dtype=np.complex128)
+        self._check(load_library().dns_ens_get_om2(self._h, m, _ptr(out)))
+        return out
+
+    def set_om2(self, m: int, a) -> None:
+        N = self.members[m][0]
+        a = np.ascontiguousarray(a, dtype=np.complex128).reshape(N, N // 2)
+        self._check(load_library().dns_ens_set_om2(self._h, m, _ptr(a)))
+
+    def fnm1(self, m: int) -> np.ndarray:
+        N = self.members[m][0]
+        out = np.empty((N, N // 2), dtype=np.complex128)
+        self._check(load_library().dns_ens_get_fnm1(self._h, m, _ptr(out)))
+        return out
+
+    def pair_info(self, m: int):
+        c, n = ctypes.c_int(), ctypes.c_int()
+        self._check(load_library().dns_ens_pair_info(self._h, m, ctypes.byref(c), ctypes.byref(n)))
+        return c.value, n.value
+
+    def pairs(self, m: int, first: int = 0, count: int = None) -> np.ndarray:
+        """Training pairs of member m: float32 array [count][4][n][n]."""
+        have, n = self.pair_info(m)
+        if count is None:
+            count = have - first
+        out = np.empty((count, 4, n, n), dtype=np.float32)
+        self._check(load_library().dns_ens_get_pairs(self._h, m, first, count,
+                                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def device_bytes(self) -> int:
+        return int(load_library().dns_ens_device_bytes(self._h))

@@ -4,7 +4,7 @@
 * ``csrc/dns_kernels.hip`` (fused kernels), ``csrc/dns_big.hip`` (multi-pass path) --
   device code, hipcc --offload-arch=gfx950 (packed FP32 ops disabled for every HIP
   translation unit); the two compile in parallel.
-* ``csrc/dns_api.cpp``, ``csrc/ic_host.cpp`` -- host code, g++ with -ffp-contract=off so
+* ``csrc/dns_api.cpp``, ``csrc/dns_ensemble.cpp``, ``csrc/ic_host.cpp`` -- host code, g++ with -ffp-contract=off so
   the FP32 parts of the reference initial condition round exactly like the reference.
 * ``driver/mem_hip.cpp`` -- the command-line driver, linked against the library.
 
@@ -44,7 +44,7 @@ def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     procs = []
-    for src in ("dns_kernels.hip", "dns_big.hip"):
+    for src in ("dns_kernels.hip", "dns_big.hip", "dns_ens_kernels.hip"):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         print("+", " ".join(cmd), flush=True)
@@ -53,7 +53,7 @@ def build(verbose: bool = True) -> str:
     for cmd, p in procs:
         if p.wait() != 0:
             raise subprocess.CalledProcessError(p.returncode, cmd)
-    for src in ("dns_api.cpp", "ic_host.cpp"):
+    for src in ("dns_api.cpp", "dns_ensemble.cpp", "ic_host.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -30,6 +30,16 @@ struct ColArgs {
 
 enum { LAUNCH_COL = 0, LAUNCH_ROW = 1, LAUNCH_ROW_CFL = 2, LAUNCH_ROW_PHYS = 3 };
 
+// Independent problems of one size advanced by one launch (blockIdx.y = member b):
+// member b's arrays start at the member-0 pointers plus b times these strides
+// (elements), and its CN/AB2 coefficients (VT, c2, c3) are read from coef[3 b ..]
+// when coef is set (the column kernel's ColArgs values otherwise).
+struct BatchArgs {
+    int count = 1;
+    size_t s_om = 0, s_v = 0, s_g = 0, s_cfl = 0;
+    const double *coef = nullptr;
+};
+
 struct LaunchReq {
     int what;
     hipStream_t stream;
@@ -49,6 +59,7 @@ struct LaunchReq {
     const dnsfft::cd *Va, *Vb;
     double *out0, *out1;
     int out_mode;
+    BatchArgs batch;           // column and row kernels (MODE 0, 1)
 };
 
 int dns_kernels_supported(int H);

@@ -101,11 +101,24 @@ __device__ __forceinline__ double inv_g_of(int iz, double gz) { return (iz >= 1 
 template <class TC, int H, int C, int T, int MINW>
 __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
                                            const TC *__restrict__ G, TC *__restrict__ V,
-                                           const TC *__restrict__ twT, const cd *__restrict__ tw, ColArgs a)
+                                           const TC *__restrict__ twT, const cd *__restrict__ tw, ColArgs a,
+                                           BatchArgs bt)
 {
     constexpr int N = 2 * H;
     constexpr int M = 3 * H;
     constexpr int PJ = C * H / T;
+    {
+        const size_t b = blockIdx.y;
+        om += b * bt.s_om;
+        fn += b * bt.s_om;
+        if (G) G += b * bt.s_g;
+        V += b * bt.s_v;
+        if (bt.coef) {
+            a.VT = bt.coef[3 * b];
+            a.c2 = bt.coef[3 * b + 1];
+            a.c3 = bt.coef[3 * b + 2];
+        }
+    }
     constexpr int LS = padded_len(H);
     static_assert((C * H) % T == 0, "tile must divide evenly over the threads");
     __shared__ TC lds[3 * C * LS];
@@ -243,8 +256,14 @@ __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__rest
 template <class TC, int H, int T, int MODE, int MINW>
 __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_t vstride, TC *__restrict__ G,
                                            const TC *__restrict__ tw, double *__restrict__ cflrow,
-                                           double DX, double DZ)
+                                           double DX, double DZ, BatchArgs bt)
 {
+    {
+        const size_t b = blockIdx.y;
+        V += b * bt.s_v;
+        if (G) G += b * bt.s_g;
+        if (cflrow) cflrow += b * bt.s_cfl;
+    }
     using RT = decltype(TC::x);
     constexpr int LS = padded_len(H);
     constexpr int PJ = (H / T) > 0 ? (H / T) : 1;  // input pairs per lane
@@ -579,20 +598,21 @@ hipError_t launch_prec(const LaunchReq &r)
     constexpr int WC = min_waves<TC, H, Q::TC, 3 * Q::C * padded_len(H) * (int)sizeof(TC)>();
     constexpr int WR = min_waves<TC, H, Q::TR, 3 * padded_len(H) * (int)sizeof(TC)>();
     const TC *twT = static_cast<const TC *>(r.twT);
+    const unsigned nb = r.batch.count > 1 ? (unsigned)r.batch.count : 1u;
     switch (r.what) {
         case LAUNCH_COL:
-            hipLaunchKernelGGL((k_col<TC, H, Q::C, Q::TC, WC>), dim3(H / Q::C), dim3(Q::TC), 0, r.stream, r.om, r.fn,
-                               static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col);
+            hipLaunchKernelGGL((k_col<TC, H, Q::C, Q::TC, WC>), dim3(H / Q::C, nb), dim3(Q::TC), 0, r.stream, r.om,
+                               r.fn, static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col, r.batch);
             break;
         case LAUNCH_ROW:
-            hipLaunchKernelGGL((k_row<TC, H, Q::TR, 0, WR>), dim3(M), dim3(Q::TR), 0, r.stream,
+            hipLaunchKernelGGL((k_row<TC, H, Q::TR, 0, WR>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
                                static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
-                               r.DX, r.DZ);
+                               r.DX, r.DZ, r.batch);
             break;
         case LAUNCH_ROW_CFL:
-            hipLaunchKernelGGL((k_row<TC, H, Q::TR, 1, WR>), dim3(M), dim3(Q::TR), 0, r.stream,
+            hipLaunchKernelGGL((k_row<TC, H, Q::TR, 1, WR>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
                                static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
-                               r.DX, r.DZ);
+                               r.DX, r.DZ, r.batch);
             break;
         case LAUNCH_ROW_PHYS:
             hipLaunchKernelGGL((k_row_phys<H, Q::TR>), dim3(M), dim3(Q::TR), 0, r.stream, r.Va, r.Vb, r.tw,

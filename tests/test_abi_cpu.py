@@ -14,9 +14,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared_symbols():
-    with open(os.path.join(ROOT, "include", "dns_hip.h"), encoding="utf-8") as f:
-        text = f.read()
-    return sorted(set(re.findall(r"\b(dns_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            with open(os.path.join(ROOT, "include", h), encoding="utf-8") as f:
+                names |= set(re.findall(r"\b(dns_[a-z0-9_]+)\s*\(", f.read()))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -88,3 +91,24 @@ def test_bad_arguments_are_rejected():
     assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 1  # unknown path
     with pytest.raises(oa.DnsError):
         oa.ic_k4exp_host(30, 4.0, 0)
+
+
+def test_ensemble_rejects_bad_configs_before_touching_a_device():
+    with pytest.raises(oa.DnsError) as ei:
+        oa.Ensemble([(100, 1e3, 4.0, 0.1, 0)])  # no fused kernels for N=100
+    assert ei.value.status == 2
+    with pytest.raises(oa.DnsError) as ei:
+        oa.Ensemble([(1024, 1e3, 4.0, 0.1, 0)], filter_ratio=8)  # coarse grid 128 > 64
+    assert ei.value.status == 2
+    with pytest.raises(oa.DnsError) as ei:
+        oa.Ensemble([(256, -1.0, 4.0, 0.1, 0)])
+    assert ei.value.status == 1
+
+
+def test_ensemble_create_fails_loudly_without_device():
+    import torch  # noqa: F401  (only to ask whether a device is present)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present; covered by the gpu tests")
+    with pytest.raises(oa.DnsError) as ei:
+        oa.Ensemble([(256, 1e3, 4.0, 0.1, 0)])
+    assert ei.value.status == 3
