@@ -12,6 +12,11 @@ The N=29400 leg (Re=1e7, K0=4, CFL=0.1, 41 steps, NEXTDT at it=1, 20, 40, 41; st
 FP64 and in mixed precision, and reports the relative L2 vorticity difference between
 the two after step 41 under "n29400".
 
+The ensemble leg ("ensemble") advances 192 runs at N=256 and 64 runs at N=512 (seeds
+0..255, K0 drawn from {4, 8, 12, 16}, Re log-uniform in [1e3, 1e5], CFL=0.1) for 2000
+steps with training pairs every 20 steps, in HIP-graph-replayed blocks, and reports
+member-steps per second.
+
 Prints one JSON line (see the keys below).  ``--gpus`` above 1 is refused: this build
 runs on one GPU.
 """
@@ -128,6 +133,66 @@ def big_leg(N: int) -> dict:
     }
 
 
+def ensemble_members():
+    """The ensemble workload: seeds 0..255 (N=256 for the first 192, N=512 for the
+    rest), K0 from {4, 8, 12, 16} and Re log-uniform in [1e3, 1e5], drawn from a
+    fixed-seed generator."""
+    import numpy as np
+
+    rng = np.random.default_rng(12345)
+    out = []
+    for seed in range(256):
+        K0 = float(rng.choice([4, 8, 12, 16]))
+        Re = float(10 ** rng.uniform(3, 5))
+        out.append((256 if seed < 192 else 512, Re, K0, 0.1, seed))
+    return out
+
+
+def ensemble_leg(steps: int = 2000) -> dict:
+    import optimizec_amd as oa
+
+    members = ensemble_members()
+    e = oa.Ensemble(members, pair_every=20, filter_ratio=8, max_pairs=steps // 20)
+    e.init()
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.advance(steps)
+    e.synchronize()
+    wall = time.perf_counter() - t0
+    count, n0 = e.pair_info(0)
+    dev = e.device_bytes()
+    s0 = e.scalars(0)
+    e.close()
+    # one member of each size alone, for the cost of running the members one by one
+    single = {}
+    for N in (256, 512):
+        m = next(x for x in members if x[0] == N)
+        s = oa.Solver(*m[:4])
+        s.init_ic(oa.IC_K4EXP, m[4])
+        s.start()
+        s.step(20)
+        s.synchronize()
+        t1 = time.perf_counter()
+        for it in range(1, 201):
+            s.step(1)
+            if it % 20 == 0:
+                s.next_dt()
+        s.synchronize()
+        single[N] = (time.perf_counter() - t1) / 200
+        s.close()
+    seq = (192 * single[256] + 64 * single[512]) * steps
+    algo = (192 * 256 ** 2 + 64 * 512 ** 2) * 152.0 * steps
+    return {"workload": "192 runs N=256 + 64 runs N=512, seeds 0..255, K0 in {4,8,12,16}, Re log-uniform "
+                        "[1e3,1e5], CFL=0.1, FP64, 2000 steps, NEXTDT and training pairs every 20 steps "
+                        "(8x sharp-spectral filter: 32^2 / 64^2)",
+            "seconds": round(wall, 3), "ms_per_step": round(1e3 * wall / steps, 4),
+            "member_steps_per_s": round(256 * steps / wall, 1),
+            "step_hbm_GBps": round(algo / wall / 1e9, 1),
+            "pairs_per_member": count, "coarse_n": [n0, 64], "device_GB": round(dev / 1e9, 2),
+            "one_by_one_estimate_s": round(seq, 2), "speedup_vs_one_by_one": round(seq / wall, 2),
+            "member0_final": {"it": s0.it, "t": s0.t, "dt": s0.dt}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +202,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--big-N", type=int, default=BIG["N"], help="large-grid leg (0 skips it)")
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
@@ -245,6 +311,8 @@ def main() -> int:
     }
     if args.big_N > 0:
         out["n29400" if args.big_N == 29400 else f"n{args.big_N}"] = big_leg(args.big_N)
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N)
     print(json.dumps(out), flush=True)
