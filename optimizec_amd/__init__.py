@@ -66,6 +66,7 @@ class EnsConfig(ctypes.Structure):
 _lib = None
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
 
 # every symbol include/dns_hip.h and include/dns_ensemble.h declare, with its ctypes signature
 SIGNATURES = {
@@ -116,6 +117,20 @@ SIGNATURES = {
     "dns_ens_get_pairs": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_float)]),
     "dns_ens_device_bytes": (ctypes.c_size_t, [_P]),
+    # closure network (include/dns_closure.h)
+    "dns_cnn_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "dns_cnn_destroy": (None, [_P]),
+    "dns_cnn_last_error": (ctypes.c_char_p, [_P]),
+    "dns_cnn_param_count": (ctypes.c_int, []),
+    "dns_cnn_get_params": (ctypes.c_int, [_P, _F]),
+    "dns_cnn_set_params": (ctypes.c_int, [_P, _F]),
+    "dns_cnn_forward": (ctypes.c_int, [_P, _F, ctypes.c_int, ctypes.c_int, _F]),
+    "dns_cnn_loss_grad": (ctypes.c_int, [_P, _F, _F, ctypes.c_int, ctypes.c_int, _D, _F]),
+    "dns_cnn_adam_step": (ctypes.c_int, [_P, _F, _F, ctypes.c_int, ctypes.c_int, ctypes.c_float, _D]),
+    "dns_cnn_train_on_ensemble": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                                 ctypes.c_uint64, _D]),
+    "dns_set_closure": (ctypes.c_int, [_P, _P]),
+    "dns_closure_forcing": (ctypes.c_int, [_P, _D]),
 }
 
 
@@ -307,6 +322,17 @@ class Solver:
     def device_bytes(self) -> int:
         return int(load_library().dns_device_bytes(self._h))
 
+    # -- closure
+    def set_closure(self, net) -> None:
+        """Attach a ClosureNet (its forcing is added in every step) or None to detach."""
+        self._closure = net
+        self._check(load_library().dns_set_closure(self._h, net._h if net is not None else None))
+
+    def closure_forcing(self) -> np.ndarray:
+        out = np.empty((self.N, self.H), dtype=np.complex128)
+        self._check(load_library().dns_closure_forcing(self._h, _ptr(out)))
+        return out
+
 
 class Ensemble:
     """Many independent runs advanced together (include/dns_ensemble.h): members given as
@@ -393,3 +419,74 @@ class Ensemble:
 
     def device_bytes(self) -> int:
         return int(load_library().dns_ens_device_bytes(self._h))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(_F)
+
+
+class ClosureNet:
+    """The subgrid-closure network (include/dns_closure.h): trained on the GPU, applied
+    in coarse runs through Solver.set_closure."""
+
+    def __init__(self, seed: int = 0, device: int = 0):
+        L = load_library()
+        h = _P()
+        st = L.dns_cnn_create(int(device), ctypes.c_uint64(seed), ctypes.byref(h))
+        if st:
+            raise DnsError(st, L.dns_cnn_last_error(None).decode())
+        self._h = h
+        self.nparams = int(L.dns_cnn_param_count())
+
+    def _check(self, st: int) -> None:
+        if st:
+            raise DnsError(st, load_library().dns_cnn_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load_library().dns_cnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparams, dtype=np.float32)
+        self._check(load_library().dns_cnn_get_params(self._h, _fptr(out)))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        self._check(load_library().dns_cnn_set_params(self._h, _fptr(p)))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        B, c, n, _ = x.shape
+        y = np.empty((B, n, n), dtype=np.float32)
+        self._check(load_library().dns_cnn_forward(self._h, _fptr(x), B, n, _fptr(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        B, _, n, _ = x.shape
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._check(load_library().dns_cnn_loss_grad(self._h, _fptr(x), _fptr(t), B, n, ctypes.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def adam_step(self, x, t, lr: float = 1e-3) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        B, _, n, _ = x.shape
+        loss = ctypes.c_double()
+        self._check(load_library().dns_cnn_adam_step(self._h, _fptr(x), _fptr(t), B, n, ctypes.c_float(lr),
+                                                     ctypes.byref(loss)))
+        return loss.value
+
+    def train_on_ensemble(self, ens, iterations: int, batch: int = 64, lr: float = 1e-3, seed: int = 0):
+        hist = np.zeros(iterations, dtype=np.float64)
+        self._check(load_library().dns_cnn_train_on_ensemble(self._h, ens._h, iterations, batch, ctypes.c_float(lr),
+                                                             ctypes.c_uint64(seed), _ptr(hist)))
+        return hist

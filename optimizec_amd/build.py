@@ -44,7 +44,7 @@ def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     procs = []
-    for src in ("dns_kernels.hip", "dns_big.hip", "dns_ens_kernels.hip"):
+    for src in ("dns_kernels.hip", "dns_big.hip", "dns_ens_kernels.hip", "cnn_kernels.hip", "closure_kernels.hip"):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         print("+", " ".join(cmd), flush=True)
@@ -53,7 +53,7 @@ def build(verbose: bool = True) -> str:
     for cmd, p in procs:
         if p.wait() != 0:
             raise subprocess.CalledProcessError(p.returncode, cmd)
-    for src in ("dns_api.cpp", "dns_ensemble.cpp", "ic_host.cpp"):
+    for src in ("dns_api.cpp", "dns_ensemble.cpp", "dns_closure.cpp", "ic_host.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -20,7 +20,9 @@
 #include <string>
 #include <vector>
 
+#include "../../include/dns_closure.h"
 #include "../../include/dns_hip.h"
+#include "closure_internal.h"
 #include "dns_big.h"
 #include "dns_kernels.h"
 
@@ -66,6 +68,10 @@ struct dns_handle {
     void *Rb = nullptr;         // multi-pass: 2 row planes [p][z][x]
     void *Sb = nullptr;         // multi-pass: line-FFT scratch; also diagnostics plane and staging
     size_t zb_bytes = 0, rb_bytes = 0, sb_bytes = 0;
+    dns_cnn *closure = nullptr;  // network whose forcing each step adds (fused FP64 path)
+    cd *twN = nullptr, *caz = nullptr, *cbz = nullptr, *fext = nullptr;
+    float *cfields = nullptr;
+    double *crow = nullptr, *cscale = nullptr;
     dns_scalars sc{};
     hipStream_t stream = nullptr;
     cd *om = nullptr, *fn = nullptr, *tw = nullptr;
@@ -507,6 +513,19 @@ dns_status download_spec(dns_handle *h, const cd *src, double *host)
     return DNS_OK;
 }
 
+// subgrid forcing of the attached network for the current om into h->fext
+dns_status closure_forcing(dns_handle *h)
+{
+    const int N = h->N;
+    HIPCHK(h, closure_launch_to_grid(h->om, N, h->twN, h->caz, h->cfields, h->crow, h->stream));
+    HIPCHK(h, closure_launch_pack(h->cfields, h->crow, N, h->cscale, cnn_input_buffer(h->closure), h->stream));
+    if (cnn_forward_device(h->closure, 1, N, h->stream) != DNS_OK)
+        return fail(h, DNS_E_HIP, "closure network: %s", cnn_error(h->closure));
+    HIPCHK(h, closure_launch_from_grid(cnn_output_buffer(h->closure), h->cscale, N, h->twN, h->cbz, h->fext,
+                                       h->stream));
+    return DNS_OK;
+}
+
 void reset_scalars(dns_handle *h)
 {
     // mem_cuda.cu:1755-1765
@@ -663,8 +682,9 @@ extern "C" void dns_destroy(dns_handle *h)
         h->Vd = nullptr;
         h->phys = nullptr;
     }
-    void *ptrs[] = {h->om, h->fn, h->V,     h->G,      h->tw,  h->twT, h->Vd, h->phys,
-                    h->cflrow, h->partEZ, h->red, h->redu, h->Zb, h->Rb, h->Sb};
+    void *ptrs[] = {h->om,  h->fn,     h->V,   h->G,    h->tw,  h->twT, h->Vd,   h->phys,  h->cflrow,
+                    h->partEZ, h->red, h->redu, h->Zb,   h->Rb,  h->Sb,  h->twN, h->caz, h->cbz,
+                    h->fext, h->cfields, h->crow, h->cscale};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -730,6 +750,10 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
             h->sc.cnm1 = h->sc.cn;
             continue;
         }
+        if (h->closure) {
+            dns_status st = closure_forcing(h);
+            if (st != DNS_OK) return st;
+        }
         LaunchReq r{};
         r.what = LAUNCH_ROW;
         r.fp32 = h->fp32;
@@ -759,6 +783,7 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
         c.col.VT = 0.5 * h->sc.visc * dt;
         c.col.c2 = 0.5 * dt * (2.0 + h->sc.cnm1);
         c.col.c3 = -0.5 * dt * h->sc.cnm1;
+        c.col.fext = h->closure ? h->fext : nullptr;
         st = launch(h, KT_COL, c);
         if (st != DNS_OK) return st;
         h->sc.t += dt;
@@ -1138,3 +1163,53 @@ extern "C" dns_status dns_synchronize(dns_handle *h)
 }
 
 extern "C" size_t dns_device_bytes(const dns_handle *h) { return h ? h->bytes : 0; }
+
+// =====================================================================================
+// closure (include/dns_closure.h)
+// =====================================================================================
+extern "C" dns_status dns_set_closure(dns_handle *h, dns_cnn *c)
+{
+    if (!h) return DNS_E_ARG;
+    if (!c) {
+        h->closure = nullptr;
+        return DNS_OK;
+    }
+    if (h->big || h->fp32) return fail(h, DNS_E_UNSUPPORTED, "the closure runs on the FP64 fused path only");
+    if (!closure_grid_supported(h->N)) return fail(h, DNS_E_UNSUPPORTED, "no closure transforms for N=%d", h->N);
+    if (cnn_device(c) != h->device) return fail(h, DNS_E_ARG, "network and solver are on different devices");
+    HIPCHK(h, hipSetDevice(h->device));
+    const int N = h->N, H = h->H;
+    if (!h->twN) {
+        dns_status st;
+        if ((st = dev_alloc(h, (void **)&h->twN, sizeof(cd) * N, "closure twiddles")) != DNS_OK) return st;
+        if ((st = dev_alloc(h, (void **)&h->caz, sizeof(cd) * 3 * (size_t)N * H, "closure z-lines")) != DNS_OK)
+            return st;
+        if ((st = dev_alloc(h, (void **)&h->cbz, sizeof(cd) * (size_t)N * H, "closure rows")) != DNS_OK) return st;
+        if ((st = dev_alloc(h, (void **)&h->fext, sizeof(cd) * (size_t)N * H, "closure forcing")) != DNS_OK)
+            return st;
+        if ((st = dev_alloc(h, (void **)&h->cfields, sizeof(float) * 3 * (size_t)N * N, "closure fields")) != DNS_OK)
+            return st;
+        if ((st = dev_alloc(h, (void **)&h->crow, sizeof(double) * N, "closure row sums")) != DNS_OK) return st;
+        if ((st = dev_alloc(h, (void **)&h->cscale, sizeof(double) * 4, "closure scales")) != DNS_OK) return st;
+        std::vector<cd> tw(N);
+        for (int t = 0; t < N; ++t) {
+            const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)N;
+            tw[t] = cd{(double)cosl(a), -(double)sinl(a)};
+        }
+        HIPCHK(h, hipMemcpy(h->twN, tw.data(), sizeof(cd) * N, hipMemcpyHostToDevice));
+    }
+    if (cnn_reserve(c, (size_t)N * N) != DNS_OK) return fail(h, DNS_E_NOMEM, "closure network: %s", cnn_error(c));
+    HIPCHK(h, hipDeviceSynchronize());
+    h->closure = c;
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_closure_forcing(dns_handle *h, double *host)
+{
+    if (!h || !host) return DNS_E_ARG;
+    if (!h->closure) return fail(h, DNS_E_STATE, "no closure network attached");
+    HIPCHK(h, hipSetDevice(h->device));
+    dns_status st = closure_forcing(h);
+    if (st != DNS_OK) return st;
+    return download_spec(h, h->fext, host);
+}

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/dns_ensemble.h"
+#include "dns_ens_internal.h"
 #include "dns_ens_kernels.h"
 #include "dns_kernels.h"
 
@@ -578,3 +579,16 @@ extern "C" dns_status dns_ens_get_pairs(dns_ens *e, int m, int first, int count,
 }
 
 extern "C" size_t dns_ens_device_bytes(const dns_ens *e) { return e ? e->bytes : 0; }
+
+int dns_ens_group_count(const dns_ens *e) { return e ? (int)e->groups.size() : 0; }
+
+int dns_ens_group_pairs(const dns_ens *e, int g, EnsPairView *out)
+{
+    if (!e || !out || g < 0 || g >= (int)e->groups.size()) return 1;
+    const Group &gr = e->groups[g];
+    out->pairs = gr.pairs;
+    out->count = gr.count;
+    out->stored = std::min(e->pairs_emitted, e->max_pairs);
+    out->n = gr.n;
+    return 0;
+}

@@ -26,6 +26,7 @@ struct ColArgs {
     double divxz;       // 1 / M^2
     double VT;          // visc * dt / 2
     double c2, c3;      // dt (2 + cnm1) / 2, -dt cnm1 / 2
+    const dnsfft::cd *fext;  // extra forcing added to FN, layout of om (single problem), or null
 };
 
 enum { LAUNCH_COL = 0, LAUNCH_ROW = 1, LAUNCH_ROW_CFL = 2, LAUNCH_ROW_PHYS = 3 };

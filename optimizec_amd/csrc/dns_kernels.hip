@@ -164,8 +164,9 @@ __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__rest
                 const double K2 = ax * ax + gz * gz;
                 const cd X = half ? xhi[j] : xlo[j];
                 cd FN = mk(-X.x * a.divxz, -X.y * a.divxz);
-                if (iz == H) FN = mk(0.0, 0.0);
                 const size_t io = (size_t)kx * N + iz;
+                if (a.fext) FN = add(FN, a.fext[io]);
+                if (iz == H) FN = mk(0.0, 0.0);
                 const cd o = om[io], f = fn[io];
                 const double ARG = a.VT * K2;
                 const double DEN = 1.0 + ARG;

@@ -1,0 +1,148 @@
+"""GPU tests of the closure network (include/dns_closure.h) against the FP64 CPU oracle
+(oracle/cnn_oracle.py).
+
+Tolerances: the kernels compute in BF16 (8-bit significand) with FP32 accumulation, so
+forward outputs and gradients agree with the FP64 oracle to about 1e-2 relative; the
+Adam update is FP32 and is compared with the oracle's update of the same gradient to
+1e-6.  The closure forcing inside a solver is compared with the same network applied
+to the fields computed on the host (numpy inverse FFT).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import optimizec_amd as oa
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import cnn_oracle as co  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def _data(seed, B=4, n=16):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((B, 3, n, n)).astype(np.float32)
+    t = rng.standard_normal((B, n, n)).astype(np.float32) * 0.5
+    return x, t
+
+
+def test_forward_matches_oracle():
+    print("closure net forward vs oracle", flush=True)
+    net = oa.ClosureNet(seed=1)
+    p = net.params().astype(np.float64)
+    assert p.size == co.param_count()
+    x, _ = _data(0)
+    y = net.forward(x)
+    yo, _ = co.forward(p, x.astype(np.float64))
+    err = rel(y, yo)
+    print(f"forward rel err {err:.2e}")
+    assert err < 2e-2
+    net.close()
+
+
+def test_gradient_matches_oracle():
+    print("closure net gradient vs oracle", flush=True)
+    net = oa.ClosureNet(seed=2)
+    p = net.params().astype(np.float64)
+    x, t = _data(3, B=4, n=16)
+    loss, g = net.loss_grad(x, t)
+    lo, go = co.loss_and_grad(p, x.astype(np.float64), t.astype(np.float64))
+    print(f"loss {loss:.6f} oracle {lo:.6f}; gradient rel err {rel(g, go):.2e}")
+    assert abs(loss / lo - 1) < 1e-2
+    assert rel(g, go) < 3e-2
+    off = 0
+    for ci, cout in co.LAYERS:
+        nw = cout * ci * 9
+        assert rel(g[off:off + nw], go[off:off + nw]) < 5e-2, (ci, cout)
+        off += nw + cout
+    net.close()
+
+
+def test_adam_step_matches_oracle_update():
+    print("closure net Adam vs oracle", flush=True)
+    net = oa.ClosureNet(seed=4)
+    p0 = net.params().astype(np.float64)
+    x, t = _data(5)
+    _, g = net.loss_grad(x, t)
+    net.adam_step(x, t, lr=1e-3)
+    p1 = net.params().astype(np.float64)
+    want, _, _ = co.adam(p0, g.astype(np.float64), np.zeros_like(p0), np.zeros_like(p0), 1, lr=1e-3)
+    assert np.max(np.abs(p1 - want)) < 1e-6
+    net.close()
+
+
+def test_training_on_ensemble_pairs_reduces_loss():
+    print("closure net training on ensemble pairs", flush=True)
+    members = [(256, 10 ** (3 + 0.25 * i), [4, 8, 12, 16][i % 4], 0.1, i) for i in range(8)]
+    e = oa.Ensemble(members, pair_every=10, filter_ratio=8, max_pairs=6)
+    e.init()
+    e.advance(60)
+    net = oa.ClosureNet(seed=7)
+    hist = net.train_on_ensemble(e, iterations=300, batch=16, lr=1e-3, seed=11)
+    first, last = hist[:20].mean(), hist[-20:].mean()
+    print(f"loss first 20 {first:.4f}, last 20 {last:.4f}")
+    assert np.all(np.isfinite(hist))
+    assert last < 0.8 * first
+    net.close()
+    e.close()
+
+
+def _phys(S, n):
+    full = np.zeros((n, n // 2 + 1), dtype=np.complex128)
+    full[:, : n // 2] = S
+    return np.fft.irfft2(full, s=(n, n)) * (n * n)
+
+
+def test_closure_forcing_matches_host_pipeline():
+    print("closure forcing in a solver vs host pipeline", flush=True)
+    N = 64
+    net = oa.ClosureNet(seed=3)
+    s = oa.Solver(N, 1e4, 4.0, 0.1)
+    s.init_ic(oa.IC_K4EXP, 5)
+    s.set_closure(net)
+    f = s.closure_forcing()
+    om = s.om2()
+    H = N // 2
+    kx = np.arange(H)[None, :].astype(float)
+    kz = np.where(np.arange(N) < H, np.arange(N), np.arange(N) - N)[:, None].astype(float)
+    k2 = kx ** 2 + kz ** 2
+    su = np.where(kx >= 1, -kz / np.where(k2 > 0, k2, 1.0), np.where(kz != 0, -1.0 / np.where(kz != 0, kz, 1.0), 0.0))
+    sw = np.where(kx >= 1, kx / np.where(k2 > 0, k2, 1.0), 0.0)
+    w, u, v = _phys(om, N), _phys(1j * su * om, N), _phys(1j * sw * om, N)
+    U = np.sqrt(np.mean(u * u + v * v))
+    D = 2 * np.pi / N
+    x = np.stack([w * D / U, u / U, v / U])[None].astype(np.float32)
+    tau = net.forward(x)[0].astype(np.float64) * U * U / (D * D)
+    spec = np.fft.fft2(tau) / (N * N)
+    want = spec[:, :H].copy()
+    want[H] = 0
+    err = rel(f, want)
+    print(f"closure forcing rel err {err:.2e}")
+    assert err < 1e-2
+    s.close()
+    net.close()
+
+
+def test_zero_network_leaves_the_run_unchanged():
+    print("closure with zero weights == plain run", flush=True)
+    N = 64
+    net = oa.ClosureNet(seed=0)
+    net.set_params(np.zeros(net.nparams, dtype=np.float32))
+    a = oa.Solver(N, 1e4, 4.0, 0.1)
+    b = oa.Solver(N, 1e4, 4.0, 0.1)
+    for s in (a, b):
+        s.init_ic(oa.IC_K4EXP, 2)
+        s.start()
+    b.set_closure(net)
+    a.run(20, 10)
+    b.run(20, 10)
+    assert np.array_equal(a.om2(), b.om2())
+    for s in (a, b):
+        s.close()
+    net.close()
