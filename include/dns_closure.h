@@ -54,6 +54,10 @@ dns_status dns_cnn_train_on_ensemble(dns_cnn *c, dns_ens *e, int iterations, int
  * forcing of the current state (fields on the N x N grid, per-step velocity scale) to
  * the nonlinear term before the CN/AB2 update.  FP64 fused path only; NULL detaches. */
 dns_status dns_set_closure(dns_handle *h, dns_cnn *c);
+/* Wavenumber band of the forcing: each mode is weighted by 0 for |k| <= k_lo, 1 for
+ * |k| >= k_hi, linearly between (default 0, 0: all modes); the mean vorticity mode and
+ * the kz = -N/2 row never receive forcing (the subgrid term is a divergence). */
+dns_status dns_set_closure_band(dns_handle *h, double k_lo, double k_hi);
 /* The forcing the attached network gives for the current state, [iz][ix] like om2. */
 dns_status dns_closure_forcing(dns_handle *h, double *host);
 

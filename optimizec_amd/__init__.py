@@ -131,6 +131,7 @@ SIGNATURES = {
                                                  ctypes.c_uint64, _D]),
     "dns_set_closure": (ctypes.c_int, [_P, _P]),
     "dns_closure_forcing": (ctypes.c_int, [_P, _D]),
+    "dns_set_closure_band": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double]),
 }
 
 
@@ -327,6 +328,9 @@ class Solver:
         """Attach a ClosureNet (its forcing is added in every step) or None to detach."""
         self._closure = net
         self._check(load_library().dns_set_closure(self._h, net._h if net is not None else None))
+
+    def set_closure_band(self, k_lo: float, k_hi: float) -> None:
+        self._check(load_library().dns_set_closure_band(self._h, float(k_lo), float(k_hi)))
 
     def closure_forcing(self) -> np.ndarray:
         out = np.empty((self.N, self.H), dtype=np.complex128)

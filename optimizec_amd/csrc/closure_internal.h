@@ -32,7 +32,8 @@ hipError_t closure_launch_to_grid(const dnsfft::cd *om, int N, const dnsfft::cd 
 // then the network input a0 [N^2][8] bf16 = (omega scale0, u scale1, w scale1, 0 ...)
 hipError_t closure_launch_pack(const float *fields, const double *rowsum, int N, double *scale, uint16_t *a0,
                                hipStream_t s);
-// network output y [N][N] (x scale[2]) -> spectral forcing fext [kx][iz] (H x N), kz = -N/2
-// row 0; bz: work array [N][H] complex
+// network output y [N][N] (x scale[2]) -> spectral forcing fext [kx][iz] (H x N), weighted
+// by the band ramp (0 for |k| <= k_lo, 1 for |k| >= k_hi); the mean and the kz = -N/2 row
+// are 0; bz: work array [N][H] complex
 hipError_t closure_launch_from_grid(const float *y, const double *scale, int N, const dnsfft::cd *twN,
-                                    dnsfft::cd *bz, dnsfft::cd *fext, hipStream_t s);
+                                    dnsfft::cd *bz, dnsfft::cd *fext, double k_lo, double k_hi, hipStream_t s);

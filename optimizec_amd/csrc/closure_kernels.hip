@@ -165,7 +165,7 @@ __global__ __launch_bounds__(T) void k_from_grid_x(const float *__restrict__ y, 
 
 template <int N, int T>
 __global__ __launch_bounds__(T) void k_from_grid_z(const cd *__restrict__ bz, const cd *__restrict__ twN,
-                                                   cd *__restrict__ fext)
+                                                   cd *__restrict__ fext, double k_lo, double k_hi)
 {
     constexpr int H = N / 2, LS = padded_len(N);
     __shared__ cd lds[kNL * LS];
@@ -179,8 +179,15 @@ __global__ __launch_bounds__(T) void k_from_grid_z(const cd *__restrict__ bz, co
     const double inv = 1.0 / (double(N) * double(N));
     for (int e = threadIdx.x; e < kNL * N; e += T) {
         const int c = e / N, iz = e - c * N;
+        const int kx = kx0 + c;
+        const double gz = (iz < H) ? double(iz) : double(iz - N);
+        const double k = sqrt(double(kx) * kx + gz * gz);
+        // band weight: 0 up to k_lo, 1 from k_hi, linear between; the mean (k = 0) and the
+        // kz = -N/2 row always 0
+        double wgt = (k <= k_lo) ? 0.0 : (k >= k_hi ? 1.0 : (k - k_lo) / (k_hi - k_lo));
+        if (iz == H || (kx == 0 && iz == 0)) wgt = 0.0;
         const cd v = lds[c * LS + pidx(iz)];
-        fext[(size_t)(kx0 + c) * N + iz] = (iz == H) ? cd{0.0, 0.0} : cd{v.x * inv, v.y * inv};
+        fext[(size_t)kx * N + iz] = cd{v.x * inv * wgt, v.y * inv * wgt};
     }
 }
 
@@ -196,13 +203,14 @@ hipError_t to_grid(const cd *om, const cd *twN, cd *az, float *fields, double *r
 }
 
 template <int N>
-hipError_t from_grid(const float *y, const double *scale, const cd *twN, cd *bz, cd *fext, hipStream_t s)
+hipError_t from_grid(const float *y, const double *scale, const cd *twN, cd *bz, cd *fext, double k_lo, double k_hi,
+                     hipStream_t s)
 {
     constexpr int T = N >= 256 ? 256 : 64;
     hipLaunchKernelGGL((k_from_grid_x<N, T>), dim3(N / 2), dim3(T), 0, s, y, scale, twN, bz);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_from_grid_z<N, T>), dim3(N / 2 / kNL), dim3(T), 0, s, bz, twN, fext);
+    hipLaunchKernelGGL((k_from_grid_z<N, T>), dim3(N / 2 / kNL), dim3(T), 0, s, bz, twN, fext, k_lo, k_hi);
     return hipGetLastError();
 }
 
@@ -243,15 +251,15 @@ hipError_t closure_launch_pack(const float *fields, const double *rowsum, int N,
 }
 
 hipError_t closure_launch_from_grid(const float *y, const double *scale, int N, const cd *twN, cd *bz, cd *fext,
-                                    hipStream_t s)
+                                    double k_lo, double k_hi, hipStream_t s)
 {
     switch (N) {
-        case 32: return from_grid<32>(y, scale, twN, bz, fext, s);
-        case 64: return from_grid<64>(y, scale, twN, bz, fext, s);
-        case 128: return from_grid<128>(y, scale, twN, bz, fext, s);
-        case 256: return from_grid<256>(y, scale, twN, bz, fext, s);
-        case 512: return from_grid<512>(y, scale, twN, bz, fext, s);
-        case 1024: return from_grid<1024>(y, scale, twN, bz, fext, s);
+        case 32: return from_grid<32>(y, scale, twN, bz, fext, k_lo, k_hi, s);
+        case 64: return from_grid<64>(y, scale, twN, bz, fext, k_lo, k_hi, s);
+        case 128: return from_grid<128>(y, scale, twN, bz, fext, k_lo, k_hi, s);
+        case 256: return from_grid<256>(y, scale, twN, bz, fext, k_lo, k_hi, s);
+        case 512: return from_grid<512>(y, scale, twN, bz, fext, k_lo, k_hi, s);
+        case 1024: return from_grid<1024>(y, scale, twN, bz, fext, k_lo, k_hi, s);
         default: return hipErrorInvalidValue;
     }
 }

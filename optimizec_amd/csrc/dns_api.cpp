@@ -72,6 +72,7 @@ struct dns_handle {
     cd *twN = nullptr, *caz = nullptr, *cbz = nullptr, *fext = nullptr;
     float *cfields = nullptr;
     double *crow = nullptr, *cscale = nullptr;
+    double band_lo = 0.0, band_hi = 0.0;  // closure forcing band (0, 0: every mode but the mean)
     dns_scalars sc{};
     hipStream_t stream = nullptr;
     cd *om = nullptr, *fn = nullptr, *tw = nullptr;
@@ -522,7 +523,7 @@ dns_status closure_forcing(dns_handle *h)
     if (cnn_forward_device(h->closure, 1, N, h->stream) != DNS_OK)
         return fail(h, DNS_E_HIP, "closure network: %s", cnn_error(h->closure));
     HIPCHK(h, closure_launch_from_grid(cnn_output_buffer(h->closure), h->cscale, N, h->twN, h->cbz, h->fext,
-                                       h->stream));
+                                       h->band_lo, h->band_hi, h->stream));
     return DNS_OK;
 }
 
@@ -1212,4 +1213,13 @@ extern "C" dns_status dns_closure_forcing(dns_handle *h, double *host)
     dns_status st = closure_forcing(h);
     if (st != DNS_OK) return st;
     return download_spec(h, h->fext, host);
+}
+
+extern "C" dns_status dns_set_closure_band(dns_handle *h, double k_lo, double k_hi)
+{
+    if (!h) return DNS_E_ARG;
+    if (!(k_lo >= 0.0) || !(k_hi >= k_lo)) return fail(h, DNS_E_ARG, "need 0 <= k_lo <= k_hi");
+    h->band_lo = k_lo;
+    h->band_hi = k_hi;
+    return DNS_OK;
 }
