@@ -12,8 +12,11 @@
  * Parameter vector: for each layer W[co][ci][ky][kx] then b[co] (FP32); the layer
  * computes out[y][x] = b + sum W[ky][kx] in[(y+ky-1) mod n][(x+kx-1) mod n].
  * Network input x: [B][3][n][n] FP32, output / target: [B][n][n] FP32, in the network's
- * units: with U the rms velocity of the sample and D = 2 pi / n the grid spacing,
- * channels (omega D / U, u / U, w / U) and subgrid term tau D^2 / U^2.
+ * units: with f' = f minus its mean over the 5 x 5 periodic box of grid points around the
+ * point (the small-scale part of a field), w_s = rms of omega' and D = 2 pi / n the grid
+ * spacing, channels (omega' / w_s, u' / (w_s D), w' / (w_s D)) and subgrid term
+ * tau / w_s^2.  The subgrid term of near-cutoff interactions scales as w_s^2, so a grid
+ * whose cutoff carries little vorticity receives a small forcing.
  */
 #ifndef DNS_CLOSURE_H
 #define DNS_CLOSURE_H
@@ -58,6 +61,12 @@ dns_status dns_set_closure(dns_handle *h, dns_cnn *c);
  * |k| >= k_hi, linearly between (default 0, 0: all modes); the mean vorticity mode and
  * the kz = -N/2 row never receive forcing (the subgrid term is a divergence). */
 dns_status dns_set_closure_band(dns_handle *h, double k_lo, double k_hi);
+/* Dissipative limiter (default on, nu_dt_max 0.5): each mode's forcing is replaced by
+ * -nu omega-hat, nu = the damping rate of the network's forcing projected on -omega-hat,
+ * clipped to 0 <= nu dt <= nu_dt_max -- no backscatter and no explicit damping beyond
+ * the AB2 step's stable range, so a network outside its training range degrades to the
+ * plain run instead of destabilizing it.  enabled = 0 applies the raw forcing. */
+dns_status dns_set_closure_limiter(dns_handle *h, int enabled, double nu_dt_max);
 /* The forcing the attached network gives for the current state, [iz][ix] like om2. */
 dns_status dns_closure_forcing(dns_handle *h, double *host);
 

@@ -132,6 +132,7 @@ SIGNATURES = {
     "dns_set_closure": (ctypes.c_int, [_P, _P]),
     "dns_closure_forcing": (ctypes.c_int, [_P, _D]),
     "dns_set_closure_band": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double]),
+    "dns_set_closure_limiter": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double]),
 }
 
 
@@ -331,6 +332,9 @@ class Solver:
 
     def set_closure_band(self, k_lo: float, k_hi: float) -> None:
         self._check(load_library().dns_set_closure_band(self._h, float(k_lo), float(k_hi)))
+
+    def set_closure_limiter(self, enabled: bool, nu_dt_max: float = 0.5) -> None:
+        self._check(load_library().dns_set_closure_limiter(self._h, 1 if enabled else 0, float(nu_dt_max)))
 
     def closure_forcing(self) -> np.ndarray:
         out = np.empty((self.N, self.H), dtype=np.complex128)

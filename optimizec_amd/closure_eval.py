@@ -5,10 +5,14 @@
 3. A DNS at N_f = ratio * N_c and two coarse runs at N_c from the same seed (the k^4 exp
    initial condition of the coarse run is the truncation of the fine one) advance to
    the same time with a fixed time step (dt_c = ratio * dt_f, the DNS's initial CFL
-   step): one coarse run plain (the fallback), one with the closure attached.
+   step):
+   one coarse run plain (the fallback), one with the closure attached (band
+   N_c/8 .. N_c/4, dissipative limiter on), one with the raw network forcing (band
+   only).
 4. Reported: the shell energy spectra of the coarse runs against the spectrally
    truncated DNS (relative error per shell k = 1 .. N_c/2 - 1), the enstrophy
-   history, and the GPU time of each run.
+   history, and the GPU time of each run.  --dns-cache keeps the DNS reference in an
+   .npz file (computed once, reused by later evaluations of the same workload).
 
     python -m optimizec_amd.closure_eval --coarse 1024 --ratio 8 --Re 1e6 --K0 8 \\
         --coarse-steps 5000 --ensemble config
@@ -21,6 +25,7 @@ from __future__ import annotations
 import argparse
 import json
 import math
+import os
 import sys
 import time
 
@@ -73,7 +78,7 @@ def ensemble_members(kind: str, n_coarse: int, ratio: int):
 
 def evaluate(coarse: int = 64, ratio: int = 8, Re: float = 1e5, K0: float = 8.0, seed: int = 1000,
              coarse_steps: int = 357, ensemble: str = "small", train_iters: int = 1500, batch: int = 64,
-             band=None, samples: int = 5, log=None) -> dict:
+             band=None, samples: int = 5, log=None, dns_cache: str = None) -> dict:
     def say(msg):
         if log:
             print(msg, file=log, flush=True)
@@ -96,32 +101,47 @@ def evaluate(coarse: int = 64, ratio: int = 8, Re: float = 1e5, K0: float = 8.0,
     say(f"training: {train_iters} Adam steps in {t_train:.1f} s, loss {hist[:20].mean():.4f} -> {hist[-50:].mean():.4f}")
 
     every = max(1, coarse_steps // samples)
-    # DNS, fixed step
-    f = oa.Solver(Nf, Re, K0, 0.1)
-    f.init_ic(oa.IC_K4EXP, seed)
-    f.start()
-    dt_f = f.scalars().dt
-    z_dns = []
-    f.synchronize()
-    t0 = time.perf_counter()
-    for s in range(1, coarse_steps + 1):
-        f.step(ratio)
-        if s % every == 0 or s == coarse_steps:
-            f.synchronize()
-            z_dns.append((s, enstrophy(truncate(f.om2(), Nf, Nc))))
-            say(f"DNS N={Nf}: coarse step {s}/{coarse_steps}, {time.perf_counter() - t0:.1f} s")
-        elif s % 250 == 0:
-            f.synchronize()
-            say(f"DNS N={Nf}: coarse step {s}/{coarse_steps}, {time.perf_counter() - t0:.1f} s")
-    f.synchronize()
-    t_dns = time.perf_counter() - t0
-    t_end = f.scalars().t
-    E_dns = shell_spectrum(truncate(f.om2(), Nf, Nc), Nc)
-    f.close()
+    key = f"{Nc}_{ratio}_{Re:g}_{K0:g}_{seed}_{coarse_steps}_{samples}"
+    cached = None
+    if dns_cache and os.path.exists(dns_cache):
+        d = np.load(dns_cache)
+        if str(d["key"]) == key:
+            cached = d
+    if cached is not None:
+        dt_f, t_end, t_dns = float(cached["dt_f"]), float(cached["t_end"]), float(cached["t_dns"])
+        E_dns = cached["E_dns"]
+        z_dns = [(int(a), float(b)) for a, b in cached["z_dns"]]
+        say(f"DNS reference from {dns_cache}")
+    else:
+        # DNS, fixed step
+        f = oa.Solver(Nf, Re, K0, 0.1)
+        f.init_ic(oa.IC_K4EXP, seed)
+        f.start()
+        dt_f = f.scalars().dt
+        z_dns = []
+        f.synchronize()
+        t0 = time.perf_counter()
+        for s in range(1, coarse_steps + 1):
+            f.step(ratio)
+            if s % every == 0 or s == coarse_steps:
+                f.synchronize()
+                z_dns.append((s, enstrophy(truncate(f.om2(), Nf, Nc))))
+                say(f"DNS N={Nf}: coarse step {s}/{coarse_steps}, {time.perf_counter() - t0:.1f} s")
+            elif s % 250 == 0:
+                f.synchronize()
+                say(f"DNS N={Nf}: coarse step {s}/{coarse_steps}, {time.perf_counter() - t0:.1f} s")
+        f.synchronize()
+        t_dns = time.perf_counter() - t0
+        t_end = f.scalars().t
+        E_dns = shell_spectrum(truncate(f.om2(), Nf, Nc), Nc)
+        f.close()
+        if dns_cache:
+            np.savez(dns_cache, key=key, dt_f=dt_f, t_end=t_end, t_dns=t_dns, E_dns=E_dns,
+                     z_dns=np.array(z_dns, dtype=np.float64))
 
     k = np.arange(1, Nc // 2)
     runs = {}
-    for name, use in (("plain", False), ("closure", True)):
+    for name, use, limit in (("plain", False, False), ("closure", True, True), ("closure_raw", True, False)):
         c = oa.Solver(Nc, Re, K0, 0.1)
         c.init_ic(oa.IC_K4EXP, seed)
         sc = c.scalars()
@@ -130,6 +150,7 @@ def evaluate(coarse: int = 64, ratio: int = 8, Re: float = 1e5, K0: float = 8.0,
         if use:
             c.set_closure(net)
             c.set_closure_band(*band)
+            c.set_closure_limiter(limit)
         z_hist = []
         c.synchronize()
         t0 = time.perf_counter()
@@ -142,6 +163,8 @@ def evaluate(coarse: int = 64, ratio: int = 8, Re: float = 1e5, K0: float = 8.0,
         t_run = time.perf_counter() - t0
         E = shell_spectrum(c.om2(), Nc)
         r = (E[k] - E_dns[k]) / E_dns[k]
+        if not np.all(np.isfinite(E)):
+            r = np.full(k.size, np.inf)
         zr = [abs(a / b[1] - 1.0) for a, b in zip(z_hist, z_dns)]
         runs[name] = {
             "seconds": round(t_run, 3),
@@ -149,7 +172,7 @@ def evaluate(coarse: int = 64, ratio: int = 8, Re: float = 1e5, K0: float = 8.0,
             "spectrum_rel_err_max": float(np.max(np.abs(r))),
             "spectrum_rel_err_upper_half_rms": float(np.sqrt(np.mean(r[len(r) // 2:] ** 2))),
             "spectrum_energy_weighted_err": float(np.linalg.norm(E[k] - E_dns[k]) / np.linalg.norm(E_dns[k])),
-            "enstrophy_rel_err_max": float(max(zr)),
+            "enstrophy_rel_err_max": float(max(zr)) if np.all(np.isfinite(zr)) else float("inf"),
             "final_energy_ratio": float(np.sum(E[1:]) / np.sum(E_dns[1:])),
             "spectrum_rel_err_at": {int(q): float(r[q - 1]) for q in (1, 2, 4, 8, Nc // 4, Nc // 2 - 2, Nc // 2 - 1)},
         }
@@ -180,11 +203,12 @@ def main() -> int:
     ap.add_argument("--coarse-steps", type=int, default=714)
     ap.add_argument("--ensemble", choices=("small", "config"), default="small")
     ap.add_argument("--train-iters", type=int, default=1500)
+    ap.add_argument("--dns-cache", default=None)
     args = ap.parse_args()
     out = evaluate(args.coarse, args.ratio, args.Re, args.K0, args.seed, args.coarse_steps, args.ensemble,
-                   args.train_iters, log=sys.stderr)
+                   args.train_iters, log=sys.stderr, dns_cache=args.dns_cache)
     print(json.dumps(out), flush=True)
-    ok = all(math.isfinite(v["spectrum_rel_err_rms"]) for v in out["runs"].values())
+    ok = all(math.isfinite(out["runs"][r]["spectrum_rel_err_rms"]) for r in ("plain", "closure"))
     return 0 if ok else 1
 
 

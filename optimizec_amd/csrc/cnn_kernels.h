@@ -47,7 +47,6 @@ hipError_t cnn_launch_pack_layer(const float *p, size_t off, int cin, int cout, 
 hipError_t cnn_launch_pack_input(const float *x, int B, int n, uint16_t *a0, hipStream_t s);
 // training minibatch from stored pairs ([slot][member][4][n][n] FP32): sample s of
 // iteration `iter` is pair (slot, member) = hash(seed, iter, s) mod (stored x count);
-// per-sample scaling U = rms velocity, D = 2 pi / n: inputs (omega D / U, u / U, w / U),
-// target tau D^2 / U^2
+// per-sample scaling of closure_scaling.hpp (small-scale fields over their vorticity rms)
 hipError_t cnn_launch_gather_pairs(const float *pairs, int count, int stored, int n, int B, uint64_t seed, int iter,
                                    uint16_t *a0, float *t, hipStream_t s);

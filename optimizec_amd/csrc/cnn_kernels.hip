@@ -14,6 +14,7 @@
 
 #include <algorithm>
 
+#include "closure_scaling.hpp"
 #include "cnn_kernels.h"
 
 namespace {
@@ -271,30 +272,33 @@ __global__ __launch_bounds__(256) void k_gather_pairs(const float *__restrict__ 
                                                       uint64_t seed, int iter, int B, uint16_t *__restrict__ a0,
                                                       float *__restrict__ t)
 {
+    using closure_scaling::highpass;
     __shared__ double red[4];
     const int s = blockIdx.x;
     const uint64_t h = mix64(seed ^ mix64((uint64_t)iter * (uint64_t)B + (uint64_t)s));
     const uint64_t idx = h % (uint64_t)((uint64_t)stored * count);
     const int nn = n * n;
     const float *src = pairs + idx * 4 * (size_t)nn;  // pair (slot, member) = (idx / count, idx % count)
-    double u2 = 0.0;
+    double w2 = 0.0;
     for (int p = threadIdx.x; p < nn; p += 256) {
-        const double u = src[nn + p], w = src[2 * nn + p];
-        u2 += u * u + w * w;
+        const double w = highpass(src, n, p / n, p % n);
+        w2 += w * w;
     }
-    for (int o = 32; o > 0; o >>= 1) u2 += __shfl_xor(u2, o, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = u2;
+    for (int o = 32; o > 0; o >>= 1) w2 += __shfl_xor(w2, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w2;
     __syncthreads();
-    const double U = sqrt((red[0] + red[1] + red[2] + red[3]) / nn);
+    const double ws = sqrt((red[0] + red[1] + red[2] + red[3]) / nn);
     const double D = 6.283185307179586476925 / n;
-    const float iu = U > 1e-30 ? float(1.0 / U) : 1.0f;
-    const float sw = float(D) * iu, st = float(D * D) * iu * iu;
+    const float iw = ws > 1e-30 ? float(1.0 / ws) : 0.0f;
+    const float iu = ws > 1e-30 ? float(1.0 / (ws * D)) : 0.0f;
+    const float it = ws > 1e-30 ? float(1.0 / (ws * ws)) : 0.0f;
     for (int p = threadIdx.x; p < nn; p += 256) {
+        const int y = p / n, x = p % n;
         uint4 v{0u, 0u, 0u, 0u};
-        v.x = (uint32_t)f2bf(src[p] * sw) | ((uint32_t)f2bf(src[nn + p] * iu) << 16);
-        v.y = (uint32_t)f2bf(src[2 * nn + p] * iu);
+        v.x = (uint32_t)f2bf(highpass(src, n, y, x) * iw) | ((uint32_t)f2bf(highpass(src + nn, n, y, x) * iu) << 16);
+        v.y = (uint32_t)f2bf(highpass(src + 2 * (size_t)nn, n, y, x) * iu);
         *reinterpret_cast<uint4 *>(a0 + ((size_t)s * nn + p) * 8) = v;
-        t[(size_t)s * nn + p] = src[3 * nn + p] * st;
+        t[(size_t)s * nn + p] = src[3 * nn + p] * it;
     }
 }
 

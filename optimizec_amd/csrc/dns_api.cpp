@@ -70,9 +70,11 @@ struct dns_handle {
     size_t zb_bytes = 0, rb_bytes = 0, sb_bytes = 0;
     dns_cnn *closure = nullptr;  // network whose forcing each step adds (fused FP64 path)
     cd *twN = nullptr, *caz = nullptr, *cbz = nullptr, *fext = nullptr;
-    float *cfields = nullptr;
-    double *crow = nullptr, *cscale = nullptr;
+    float *cfields = nullptr, *chp = nullptr;
+    double *cpart = nullptr, *cscale = nullptr;
     double band_lo = 0.0, band_hi = 0.0;  // closure forcing band (0, 0: every mode but the mean)
+    int limiter = 1;                       // closure: dissipative limiter on (1) / raw forcing (0)
+    double nu_dt_max = 0.5;
     dns_scalars sc{};
     hipStream_t stream = nullptr;
     cd *om = nullptr, *fn = nullptr, *tw = nullptr;
@@ -518,12 +520,21 @@ dns_status download_spec(dns_handle *h, const cd *src, double *host)
 dns_status closure_forcing(dns_handle *h)
 {
     const int N = h->N;
-    HIPCHK(h, closure_launch_to_grid(h->om, N, h->twN, h->caz, h->cfields, h->crow, h->stream));
-    HIPCHK(h, closure_launch_pack(h->cfields, h->crow, N, h->cscale, cnn_input_buffer(h->closure), h->stream));
+    HIPCHK(h, closure_launch_to_grid(h->om, N, h->twN, h->caz, h->cfields, h->stream));
+    HIPCHK(h, closure_launch_pack(h->cfields, N, h->chp, h->cpart, h->cscale, cnn_input_buffer(h->closure),
+                                  h->stream));
     if (cnn_forward_device(h->closure, 1, N, h->stream) != DNS_OK)
         return fail(h, DNS_E_HIP, "closure network: %s", cnn_error(h->closure));
-    HIPCHK(h, closure_launch_from_grid(cnn_output_buffer(h->closure), h->cscale, N, h->twN, h->cbz, h->fext,
-                                       h->band_lo, h->band_hi, h->stream));
+    ClosureShape sh;
+    sh.k_lo = h->band_lo;
+    sh.k_hi = h->band_hi;
+    if (h->limiter) {
+        sh.om = h->om;
+        sh.dt = h->sc.dt;
+        sh.nu_dt_max = h->nu_dt_max;
+    }
+    HIPCHK(h, closure_launch_from_grid(cnn_output_buffer(h->closure), h->cscale, N, h->twN, h->cbz, h->fext, sh,
+                                       h->stream));
     return DNS_OK;
 }
 
@@ -685,7 +696,7 @@ extern "C" void dns_destroy(dns_handle *h)
     }
     void *ptrs[] = {h->om,  h->fn,     h->V,   h->G,    h->tw,  h->twT, h->Vd,   h->phys,  h->cflrow,
                     h->partEZ, h->red, h->redu, h->Zb,   h->Rb,  h->Sb,  h->twN, h->caz, h->cbz,
-                    h->fext, h->cfields, h->crow, h->cscale};
+                    h->fext, h->cfields, h->chp, h->cpart, h->cscale};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1190,7 +1201,11 @@ extern "C" dns_status dns_set_closure(dns_handle *h, dns_cnn *c)
             return st;
         if ((st = dev_alloc(h, (void **)&h->cfields, sizeof(float) * 3 * (size_t)N * N, "closure fields")) != DNS_OK)
             return st;
-        if ((st = dev_alloc(h, (void **)&h->crow, sizeof(double) * N, "closure row sums")) != DNS_OK) return st;
+        if ((st = dev_alloc(h, (void **)&h->chp, sizeof(float) * 3 * (size_t)N * N, "closure small scales")) != DNS_OK)
+            return st;
+        if ((st = dev_alloc(h, (void **)&h->cpart, sizeof(double) * closure_pack_blocks(), "closure partials")) !=
+            DNS_OK)
+            return st;
         if ((st = dev_alloc(h, (void **)&h->cscale, sizeof(double) * 4, "closure scales")) != DNS_OK) return st;
         std::vector<cd> tw(N);
         for (int t = 0; t < N; ++t) {
@@ -1221,5 +1236,14 @@ extern "C" dns_status dns_set_closure_band(dns_handle *h, double k_lo, double k_
     if (!(k_lo >= 0.0) || !(k_hi >= k_lo)) return fail(h, DNS_E_ARG, "need 0 <= k_lo <= k_hi");
     h->band_lo = k_lo;
     h->band_hi = k_hi;
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_set_closure_limiter(dns_handle *h, int enabled, double nu_dt_max)
+{
+    if (!h) return DNS_E_ARG;
+    if (enabled && !(nu_dt_max >= 0.0)) return fail(h, DNS_E_ARG, "nu_dt_max must be >= 0");
+    h->limiter = enabled ? 1 : 0;
+    h->nu_dt_max = nu_dt_max;
     return DNS_OK;
 }
