@@ -17,6 +17,11 @@ The ensemble leg ("ensemble") advances 192 runs at N=256 and 64 runs at N=512 (s
 steps with training pairs every 20 steps, in HIP-graph-replayed blocks, and reports
 member-steps per second.
 
+The closure leg ("closure") trains the subgrid network on the pairs of 64 runs at N=512
+and compares a coarse N=64 run with and without it against the 8x-filtered DNS at N=512
+(Re=1e5, K0=8, t=0.6); the full N=1024 vs N=8192 comparison is run by
+optimizec_amd/closure_eval.py.
+
 Prints one JSON line (see the keys below).  ``--gpus`` above 1 is refused: this build
 runs on one GPU.
 """
@@ -193,6 +198,17 @@ def ensemble_leg(steps: int = 2000) -> dict:
             "member0_final": {"it": s0.it, "t": s0.t, "dt": s0.dt}}
 
 
+def _finite(x):
+    """Non-finite floats as null, so the line is strict JSON."""
+    if isinstance(x, float):
+        return x if math.isfinite(x) else None
+    if isinstance(x, dict):
+        return {k: _finite(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [_finite(v) for v in x]
+    return x
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +219,7 @@ def main() -> int:
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--big-N", type=int, default=BIG["N"], help="large-grid leg (0 skips it)")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-closure", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
@@ -313,9 +330,19 @@ def main() -> int:
         out["n29400" if args.big_N == 29400 else f"n{args.big_N}"] = big_leg(args.big_N)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg()
+    if not args.no_closure:
+        from optimizec_amd import closure_eval
+
+        c = closure_eval.evaluate(coarse=64, ratio=8, Re=1e5, K0=8.0, seed=1000, coarse_steps=714,
+                                  ensemble="small", train_iters=1500)
+        out["closure"] = {"workload": c["workload"], "ensemble": c["ensemble"], "training": c["training"],
+                          "dns_seconds": c["dns_seconds"],
+                          "runs": {k: {q: v[q] for q in ("seconds", "spectrum_rel_err_rms", "spectrum_rel_err_max",
+                                                         "enstrophy_rel_err_max", "final_energy_ratio")}
+                                   for k, v in c["runs"].items()}}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N)
-    print(json.dumps(out), flush=True)
+    print(json.dumps(_finite(out)), flush=True)
     return 0 if finite else 1
 
 
