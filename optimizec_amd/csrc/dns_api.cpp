@@ -42,20 +42,17 @@ enum {
     KT_ROW = 0,
     KT_COL,
     KT_ROW_CFL,
-    KT_BIG_SPEC,
-    KT_BIG_ZINV,
-    KT_BIG_PACK,
-    KT_BIG_XINV,
-    KT_BIG_PROD,
-    KT_BIG_XFWD,
-    KT_BIG_UNPACK,
-    KT_BIG_ZFWD,
-    KT_BIG_CN,
+    KT_BIG_ZINV_A,
+    KT_BIG_ZINV_B,
+    KT_BIG_XINV_A,
+    KT_BIG_XINV_B,
+    KT_BIG_XFWD_B,
+    KT_BIG_ZFWD_A,
+    KT_BIG_ZFWD_B,
     KT_COUNT
 };
-const char *const kKernelNames[KT_COUNT] = {"k_row",    "k_col",     "k_row_cfl", "big_spec",
-                                            "big_zinv", "big_pack",  "big_xinv",  "big_prod",
-                                            "big_xfwd", "big_unpack", "big_zfwd", "big_cn"};
+const char *const kKernelNames[KT_COUNT] = {"k_row",      "k_col",      "k_row_cfl",  "big_zinv_a", "big_zinv_b",
+                                            "big_xinv_a", "big_xinv_b", "big_xfwd_b", "big_zfwd_a", "big_zfwd_b"};
 
 }  // namespace
 
@@ -64,10 +61,10 @@ struct dns_handle {
     int device = 0;
     int fp32 = 0;               // 1: V, G and the transforms in FP32 (state stays FP64)
     bool big = false;           // multi-pass path (dns_big.hip) instead of the fused kernels
-    void *Zb = nullptr;         // multi-pass: 4 z-line fields [f][kx][z] (transform precision)
-    void *Rb = nullptr;         // multi-pass: 2 row planes [p][z][x]
-    void *Sb = nullptr;         // multi-pass: line-FFT scratch; also diagnostics plane and staging
-    size_t zb_bytes = 0, rb_bytes = 0, sb_bytes = 0;
+    void *BA = nullptr;         // multi-pass work buffers (dns_big.h layouts): Y, X, Qh and the
+    void *BB = nullptr;         // diagnostics rows in BA; Z, Yq, W, diagnostics plane in BB
+    size_t ba_bytes = 0, bb_bytes = 0;
+    size_t hs = 0, ms = 0;      // padded row strides of the H- and M-wide rows
     dns_cnn *closure = nullptr;  // network whose forcing each step adds (fused FP64 path)
     cd *twN = nullptr, *caz = nullptr, *cbz = nullptr, *fext = nullptr;
     float *cfields = nullptr, *chp = nullptr;
@@ -194,130 +191,150 @@ dns_status launch(dns_handle *h, int timed_kernel, LaunchReq &r)
 }
 
 // ---- multi-pass path ------------------------------------------------------------
-dns_status big_launch(dns_handle *h, BigReq &r)
+dns_status big_launch(dns_handle *h, int timed_kernel, BigReq &r)
 {
     r.stream = h->stream;
     r.H = h->H;
+    r.hs = h->hs;
+    r.ms = h->ms;
+    const TimedLaunch tl = timing_begin(h, timed_kernel);
     hipError_t e = dns_big_launch(r);
     if (e != hipSuccess)
         return fail(h, DNS_E_HIP, "multi-pass kernel launch (%d) failed: %s", r.what, hipGetErrorString(e));
+    timing_end(h, tl);
     return DNS_OK;
 }
 
-// `lines` line FFTs of length M from src to dst (may be equal), in chunks that fit the
-// scratch buffer; fp64 selects FP64 elements whatever the handle's precision.
-dns_status big_fft(dns_handle *h, bool fp64, bool inverse, void *src, void *dst, size_t lines)
+// fields (pairs of DNS_F_*) of the current om on the physical rows: Y in BA, Z in BB
+dns_status big_zinv(dns_handle *h, bool fp64, const int *fields, int pairs, bool timed)
 {
-    const size_t elem = (fp64 || !h->fp32) ? sizeof(cd) : sizeof(dnsfft::cf);
-    const size_t line_bytes = elem * (size_t)h->M;
-    size_t chunk = h->sb_bytes / line_bytes;
-    if (chunk > 65535) chunk = 65535;
-    if (chunk == 0) return fail(h, DNS_E_STATE, "line-FFT scratch holds no line");
-    for (size_t l0 = 0; l0 < lines; l0 += chunk) {
-        const size_t n = std::min(chunk, lines - l0);
-        BigReq r{};
-        r.what = inverse ? BIG_FFT_INV : BIG_FFT_FWD;
-        r.fp32 = fp64 ? 0 : h->fp32;
-        r.tw = r.fp32 ? h->twT : h->tw;
-        r.a = static_cast<char *>(src) + l0 * line_bytes;
-        r.b = static_cast<char *>(dst) + l0 * line_bytes;
-        r.scratch = h->Sb;
-        r.lines = n;
-        dns_status st = big_launch(h, r);
-        if (st != DNS_OK) return st;
-    }
-    return DNS_OK;
-}
-
-// physical rows of nf spectra of the current om: fields paired as a + i b into the row
-// planes Rb (pair p at plane p); timed phases only on the step's own call (timed).
-dns_status big_inverse(dns_handle *h, bool fp64, int nf, const int *fields, bool timed)
-{
-    const int fp32 = fp64 ? 0 : h->fp32;
-    const size_t elem = fp32 ? sizeof(dnsfft::cf) : sizeof(cd);
-    const size_t nmid = (size_t)h->M * h->H, nrow = (size_t)h->M * h->M;
-    if (nf < 1 || nf > 4 || elem * nmid * nf > h->zb_bytes || elem * nrow * ((nf + 1) / 2) > h->rb_bytes)
-        return fail(h, DNS_E_STATE, "multi-pass buffers too small for %d fields", nf);
-    TimedLaunch tl = timing_begin(h, timed ? KT_BIG_SPEC : -1);
     BigReq r{};
-    r.what = BIG_SPEC;
-    r.fp32 = fp32;
+    r.what = BIG_ZINV_A;
+    r.fp32 = fp64 ? 0 : h->fp32;
+    r.tw = r.fp32 ? h->twT : h->tw;
     r.om = h->om;
-    r.a = h->Zb;
-    r.spec.nfields = nf;
-    for (int i = 0; i < nf; ++i) r.spec.fields[i] = fields[i];
-    dns_status st = big_launch(h, r);
+    r.b = h->BA;
+    r.fields.pairs = pairs;
+    for (int i = 0; i < 2 * pairs; ++i) r.fields.f[i] = fields[i];
+    dns_status st = big_launch(h, timed ? KT_BIG_ZINV_A : -1, r);
     if (st != DNS_OK) return st;
-    timing_end(h, tl);
-    tl = timing_begin(h, timed ? KT_BIG_ZINV : -1);
-    st = big_fft(h, fp64, true, h->Zb, h->Zb, (size_t)nf * h->H);
-    if (st != DNS_OK) return st;
-    timing_end(h, tl);
-    tl = timing_begin(h, timed ? KT_BIG_PACK : -1);
-    for (int p = 0; p < (nf + 1) / 2; ++p) {
-        BigReq k{};
-        k.what = BIG_PACK;
-        k.fp32 = fp32;
-        k.a = static_cast<char *>(h->Zb) + (size_t)(2 * p) * nmid * elem;
-        k.a2 = (2 * p + 1 < nf) ? static_cast<char *>(h->Zb) + (size_t)(2 * p + 1) * nmid * elem : nullptr;
-        k.b = static_cast<char *>(h->Rb) + (size_t)p * nrow * elem;
-        st = big_launch(h, k);
-        if (st != DNS_OK) return st;
-    }
-    timing_end(h, tl);
-    tl = timing_begin(h, timed ? KT_BIG_XINV : -1);
-    st = big_fft(h, fp64, true, h->Rb, h->Rb, (size_t)((nf + 1) / 2) * h->M);
-    if (st != DNS_OK) return st;
-    timing_end(h, tl);
-    return DNS_OK;
+    r.what = BIG_ZINV_B;
+    r.a = h->BA;
+    r.b = h->BB;
+    return big_launch(h, timed ? KT_BIG_ZINV_B : -1, r);
 }
 
-// one time step of the multi-pass path (same arithmetic as the fused row + column pass)
+// one time step of the multi-pass path (seven passes, dns_big.hip)
 dns_status big_step(dns_handle *h, double dt)
 {
     static const int f4[4] = {DNS_F_U, DNS_F_W, DNS_F_OX, DNS_F_OZ};
-    const size_t elem = h->fp32 ? sizeof(dnsfft::cf) : sizeof(cd);
-    const size_t nrow = (size_t)h->M * h->M;
-    dns_status st = big_inverse(h, false, 4, f4, true);
+    dns_status st = big_zinv(h, false, f4, 2, true);
     if (st != DNS_OK) return st;
-    // row-pair products into the z-line buffer (free after the pack)
-    TimedLaunch tl = timing_begin(h, KT_BIG_PROD);
+    BigReq r{};
+    r.fp32 = h->fp32;
+    r.tw = h->fp32 ? h->twT : h->tw;
+    r.what = BIG_XINV_A;
+    r.planes = 2;
+    r.a = h->BB;
+    r.b = h->BA;
+    if ((st = big_launch(h, KT_BIG_XINV_A, r)) != DNS_OK) return st;
+    r.what = BIG_XINV_B;
+    r.a = h->BA;
+    r.b = h->BB;
+    if ((st = big_launch(h, KT_BIG_XINV_B, r)) != DNS_OK) return st;
+    r.what = BIG_XFWD_B;
+    r.a = h->BB;
+    r.b = h->BA;
+    if ((st = big_launch(h, KT_BIG_XFWD_B, r)) != DNS_OK) return st;
+    r.what = BIG_ZFWD_A;
+    r.a = h->BA;
+    r.b = h->BB;
+    if ((st = big_launch(h, KT_BIG_ZFWD_A, r)) != DNS_OK) return st;
+    r.what = BIG_ZFWD_B;
+    r.a = h->BB;
+    r.om = h->om;
+    r.fn = h->fn;
+    r.cn.divxz = 1.0 / (double(h->M) * double(h->M));
+    r.cn.VT = 0.5 * h->sc.visc * dt;
+    r.cn.c2 = 0.5 * dt * (2.0 + h->sc.cnm1);
+    r.cn.c3 = -0.5 * dt * h->sc.cnm1;
+    return big_launch(h, KT_BIG_ZFWD_B, r);
+}
+
+// FP64 physical plane of one field (or |u + i w|) into h->phys, natural order:
+// fields on rows (Z in BB), rows R = a (+ i b) in BA, inverse line FFTs (scratch BB)
+dns_status big_field(dns_handle *h, int field)
+{
+    int f2[2];
+    bool two = false;
+    switch (field) {
+        case DNS_FIELD_U: f2[0] = f2[1] = DNS_F_U; break;
+        case DNS_FIELD_W: f2[0] = f2[1] = DNS_F_W; break;
+        case DNS_FIELD_OMEGA: f2[0] = f2[1] = DNS_F_OMEGA; break;
+        case DNS_FIELD_PSI: f2[0] = f2[1] = DNS_F_PSI; break;
+        case DNS_FIELD_KINETIC:
+            f2[0] = DNS_F_U;
+            f2[1] = DNS_F_W;
+            two = true;
+            break;
+        default: return fail(h, DNS_E_ARG, "unknown field %d", field);
+    }
+    dns_status st = big_zinv(h, true, f2, 1, false);
+    if (st != DNS_OK) return st;
+    BigReq r{};
+    r.what = BIG_ROWS;
+    r.tw = h->tw;
+    r.a = h->BB;
+    r.b = h->BA;
+    r.planes = two ? 2 : 1;
+    if ((st = big_launch(h, -1, r)) != DNS_OK) return st;
+    const size_t line_bytes = sizeof(cd) * (size_t)h->M;
+    size_t chunk = h->bb_bytes / line_bytes;
+    if (chunk > 65535) chunk = 65535;
+    for (size_t l0 = 0; l0 < (size_t)h->M; l0 += chunk) {
+        BigReq l{};
+        l.what = BIG_LINE_INV;
+        l.tw = h->tw;
+        l.a = static_cast<char *>(h->BA) + l0 * line_bytes;
+        l.b = static_cast<char *>(h->BA) + l0 * line_bytes;
+        l.scratch = h->BB;
+        l.lines = std::min(chunk, (size_t)h->M - l0);
+        if ((st = big_launch(h, -1, l)) != DNS_OK) return st;
+    }
     BigReq p{};
-    p.what = BIG_PROD;
-    p.fp32 = h->fp32;
-    p.a = h->Rb;
-    p.a2 = static_cast<char *>(h->Rb) + nrow * elem;
-    p.b = h->Zb;
-    if ((st = big_launch(h, p)) != DNS_OK) return st;
-    timing_end(h, tl);
-    tl = timing_begin(h, KT_BIG_XFWD);
-    if ((st = big_fft(h, false, false, h->Zb, h->Zb, (size_t)h->M / 2)) != DNS_OK) return st;
-    timing_end(h, tl);
-    // kx < H of every row, as z-lines into the row buffer (free after the product)
-    tl = timing_begin(h, KT_BIG_UNPACK);
-    BigReq u{};
-    u.what = BIG_UNPACK;
-    u.fp32 = h->fp32;
-    u.a = h->Zb;
-    u.b = h->Rb;
-    if ((st = big_launch(h, u)) != DNS_OK) return st;
-    timing_end(h, tl);
-    tl = timing_begin(h, KT_BIG_ZFWD);
-    if ((st = big_fft(h, false, false, h->Rb, h->Rb, (size_t)h->H)) != DNS_OK) return st;
-    timing_end(h, tl);
-    tl = timing_begin(h, KT_BIG_CN);
-    BigReq c{};
-    c.what = BIG_CN;
-    c.fp32 = h->fp32;
-    c.om = h->om;
-    c.fn = h->fn;
-    c.a = h->Rb;
-    c.cn.divxz = 1.0 / (double(h->M) * double(h->M));
-    c.cn.VT = 0.5 * h->sc.visc * dt;
-    c.cn.c2 = 0.5 * dt * (2.0 + h->sc.cnm1);
-    c.cn.c3 = -0.5 * dt * h->sc.cnm1;
-    if ((st = big_launch(h, c)) != DNS_OK) return st;
-    timing_end(h, tl);
+    p.what = BIG_ROWS_PHYS;
+    p.a = h->BA;
+    p.out0 = h->phys;
+    p.out_mode = two ? 1 : 0;
+    return big_launch(h, -1, p);
+}
+
+// CFLM in FP64 through passes 1-3 and the CFL form of pass 4 (any x order)
+dns_status big_cflm(dns_handle *h, double *cflm)
+{
+    const double PI = 3.14159265358979323846;
+    static const int f2[2] = {DNS_F_U, DNS_F_W};
+    dns_status st = big_zinv(h, true, f2, 1, false);
+    if (st != DNS_OK) return st;
+    BigReq r{};
+    r.tw = h->tw;
+    r.what = BIG_XINV_A;
+    r.planes = 1;
+    r.a = h->BB;
+    r.b = h->BA;
+    if ((st = big_launch(h, -1, r)) != DNS_OK) return st;
+    r.what = BIG_XINV_CFL;
+    r.a = h->BA;
+    r.cflrow = h->cflrow;
+    r.DX = r.DZ = 2.0 * PI / double(h->N);
+    if ((st = big_launch(h, -1, r)) != DNS_OK) return st;
+    const size_t n = (size_t)(h->M / 2) * dns_big_cfl_tiles(h->M);
+    std::vector<double> v(n);
+    HIPCHK(h, hipMemcpyAsync(v.data(), h->cflrow, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double m = 0.0;
+    for (double x : v) m = x > m ? x : m;
+    *cflm = m;
     return DNS_OK;
 }
 
@@ -391,17 +408,7 @@ dns_status compute_field(dns_handle *h, int field)
             break;
         default: return fail(h, DNS_E_ARG, "unknown field %d", field);
     }
-    if (h->big) {
-        st = big_inverse(h, true, nf, fields, false);
-        if (st != DNS_OK) return st;
-        BigReq p{};
-        p.what = BIG_PHYS;
-        p.fp32 = 0;
-        p.a = h->Rb;
-        p.out0 = h->phys;
-        p.out_mode = (field == DNS_FIELD_KINETIC) ? 1 : 0;
-        return big_launch(h, p);
-    }
+    if (h->big) return big_field(h, field);
     const size_t vs = (size_t)h->M * h->H;
     LaunchReq r{};
     r.what = LAUNCH_COL;
@@ -444,20 +451,8 @@ dns_status compute_cflm(dns_handle *h, double *cflm)
     r.cflrow = h->cflrow;
     r.DX = 2.0 * PI / double(h->N);
     r.DZ = 2.0 * PI / double(h->N);
-    if (h->big) {
-        static const int f2[2] = {DNS_F_U, DNS_F_W};
-        dns_status st = big_inverse(h, true, 2, f2, false);
-        if (st != DNS_OK) return st;
-        BigReq c{};
-        c.what = BIG_CFL;
-        c.fp32 = 0;
-        c.a = h->Rb;
-        c.cflrow = h->cflrow;
-        c.DX = r.DX;
-        c.DZ = r.DZ;
-        st = big_launch(h, c);
-        if (st != DNS_OK) return st;
-    } else if (!h->fp32) {
+    if (h->big) return big_cflm(h, cflm);
+    if (!h->fp32) {
         if (!h->v_valid) {
             dns_status st = refresh_v(h);
             if (st != DNS_OK) return st;
@@ -480,10 +475,8 @@ dns_status compute_cflm(dns_handle *h, double *cflm)
         if (st != DNS_OK) return st;
         r.V = h->Vd;
     }
-    if (!h->big) {
-        dns_status st = launch(h, KT_ROW_CFL, r);
-        if (st != DNS_OK) return st;
-    }
+    dns_status st = launch(h, KT_ROW_CFL, r);
+    if (st != DNS_OK) return st;
     std::vector<double> rows(h->M);
     HIPCHK(h, hipMemcpyAsync(rows.data(), h->cflrow, sizeof(double) * h->M, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -497,6 +490,11 @@ dns_status compute_cflm(dns_handle *h, double *cflm)
 dns_status upload_spec(dns_handle *h, const double *host, cd *dst)
 {
     const size_t n = (size_t)h->N * h->H;
+    if (h->big) {  // the multi-pass state is in the host layout
+        HIPCHK(h, hipMemcpyAsync(dst, host, sizeof(cd) * n, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return DNS_OK;
+    }
     dns_status st = ensure_vd(h);
     if (st != DNS_OK) return st;
     HIPCHK(h, hipMemcpyAsync(h->Vd, host, sizeof(cd) * n, hipMemcpyHostToDevice, h->stream));
@@ -508,6 +506,11 @@ dns_status upload_spec(dns_handle *h, const double *host, cd *dst)
 dns_status download_spec(dns_handle *h, const cd *src, double *host)
 {
     const size_t n = (size_t)h->N * h->H;
+    if (h->big) {
+        HIPCHK(h, hipMemcpyAsync(host, src, sizeof(cd) * n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return DNS_OK;
+    }
     dns_status st = ensure_vd(h);
     if (st != DNS_OK) return st;
     HIPCHK(h, dns_launch_transpose(src, h->Vd, h->H, h->N, h->stream));
@@ -630,25 +633,29 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     if ((st = dev_alloc(h, (void **)&h->fn, sizeof(cd) * nspec, "fnm1")) != DNS_OK) return bail(st);
     const size_t elem = h->fp32 ? sizeof(dnsfft::cf) : sizeof(cd);
     if (h->big) {
-        // z-lines: 4 fields; rows: 2 planes; scratch: one plane (the FFTs run in chunks
-        // of at most M lines).  Diagnostics run in FP64 inside the same bytes: 2 fields,
-        // 1 plane, and the FP64 physical plane / host staging in the scratch.
-        const size_t nrow = (size_t)h->M * h->M;
-        h->zb_bytes = elem * 4 * nmid;
-        h->rb_bytes = elem * 2 * nrow;
-        h->sb_bytes = std::max(elem * nrow, sizeof(double) * nrow);
-        if ((st = dev_alloc(h, &h->Zb, h->zb_bytes, "z-line fields")) != DNS_OK) return bail(st);
-        if ((st = dev_alloc(h, &h->Rb, h->rb_bytes, "row planes")) != DNS_OK) return bail(st);
-        if ((st = dev_alloc(h, &h->Sb, h->sb_bytes, "line-FFT scratch")) != DNS_OK) return bail(st);
-        h->Vd = static_cast<cd *>(h->Sb);
-        h->phys = static_cast<double *>(h->Sb);
+        // BA holds Y (4 fields), X (2 planes), Qh; BB holds Z (4 fields), Yq, W.  The FP64
+        // diagnostics use the same bytes: Y, X-plane 0 and rows R in BA; Z, line-FFT
+        // scratch and the physical plane in BB.
+        const size_t M = h->M;
+        h->hs = ((size_t)h->H + 15) / 16 * 16;
+        h->ms = (M + 15) / 16 * 16;
+        const size_t ba = std::max({elem * 2 * M * h->ms, elem * 4 * M * h->hs, sizeof(cd) * 2 * M * h->hs,
+                                    sizeof(cd) * M * h->ms, sizeof(cd) * M * M});
+        const size_t bb = std::max({elem * 4 * M * h->hs, elem * (M / 2) * h->ms, sizeof(cd) * 2 * M * h->hs,
+                                    sizeof(double) * M * M, sizeof(cd) * M * 8});
+        h->ba_bytes = ba;
+        h->bb_bytes = bb;
+        if ((st = dev_alloc(h, &h->BA, ba, "multi-pass buffer A")) != DNS_OK) return bail(st);
+        if ((st = dev_alloc(h, &h->BB, bb, "multi-pass buffer B")) != DNS_OK) return bail(st);
+        h->phys = static_cast<double *>(h->BB);
     } else {
         if ((st = dev_alloc(h, &h->V, elem * 4 * nmid, "z-inverse fields")) != DNS_OK) return bail(st);
         if ((st = dev_alloc(h, &h->G, elem * nmid, "x-forward product")) != DNS_OK) return bail(st);
     }
     if ((st = dev_alloc(h, (void **)&h->tw, sizeof(cd) * h->M, "twiddles")) != DNS_OK) return bail(st);
-    if ((st = dev_alloc(h, (void **)&h->cflrow, sizeof(double) * h->M, "cfl rows")) != DNS_OK) return bail(st);
-    if ((st = dev_alloc(h, (void **)&h->partEZ, sizeof(double) * 2 * h->H, "energy partials")) != DNS_OK)
+    const size_t ncfl = h->big ? (size_t)(h->M / 2) * dns_big_cfl_tiles(h->M) : (size_t)h->M;
+    if ((st = dev_alloc(h, (void **)&h->cflrow, sizeof(double) * ncfl, "cfl maxima")) != DNS_OK) return bail(st);
+    if ((st = dev_alloc(h, (void **)&h->partEZ, sizeof(double) * 2 * h->N, "energy partials")) != DNS_OK)
         return bail(st);
     if ((st = dev_alloc(h, (void **)&h->red, sizeof(double) * 2 * kRedBlocks, "minmax partials")) != DNS_OK)
         return bail(st);
@@ -690,12 +697,9 @@ extern "C" void dns_destroy(dns_handle *h)
     }
     for (hipEvent_t ev : h->event_pool) (void)hipEventDestroy(ev);
     if (h->twT == h->tw) h->twT = nullptr;
-    if (h->big) {  // views of the scratch buffer
-        h->Vd = nullptr;
-        h->phys = nullptr;
-    }
+    if (h->big) h->phys = nullptr;  // a view of buffer B
     void *ptrs[] = {h->om,  h->fn,     h->V,   h->G,    h->tw,  h->twT, h->Vd,   h->phys,  h->cflrow,
-                    h->partEZ, h->red, h->redu, h->Zb,   h->Rb,  h->Sb,  h->twN, h->caz, h->cbz,
+                    h->partEZ, h->red, h->redu, h->BA,   h->BB,  nullptr,  h->twN, h->caz, h->cbz,
                     h->fext, h->cfields, h->chp, h->cpart, h->cscale};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -837,14 +841,26 @@ extern "C" dns_status dns_energy_enstrophy(dns_handle *h, double *E, double *Z)
 {
     if (!h || !E || !Z) return DNS_E_ARG;
     HIPCHK(h, hipSetDevice(h->device));
-    HIPCHK(h, dns_launch_energy(h->om, h->N, h->H, h->partEZ, h->partEZ + h->H, h->stream));
-    std::vector<double> p(2 * (size_t)h->H);
+    // per-column partials (fused layout [kx][iz]) or per-row partials (multi-pass [iz][kx])
+    const int parts = h->big ? h->N : h->H;
+    if (h->big) {
+        BigReq r{};
+        r.what = BIG_ENERGY;
+        r.om = h->om;
+        r.out0 = h->partEZ;
+        r.out1 = h->partEZ + parts;
+        dns_status st = big_launch(h, -1, r);
+        if (st != DNS_OK) return st;
+    } else {
+        HIPCHK(h, dns_launch_energy(h->om, h->N, h->H, h->partEZ, h->partEZ + parts, h->stream));
+    }
+    std::vector<double> p(2 * (size_t)parts);
     HIPCHK(h, hipMemcpyAsync(p.data(), h->partEZ, sizeof(double) * p.size(), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     double e = 0.0, z = 0.0;
-    for (int i = 0; i < h->H; ++i) {
+    for (int i = 0; i < parts; ++i) {
         e += p[i];
-        z += p[h->H + i];
+        z += p[parts + i];
     }
     *E = e;
     *Z = z;
@@ -1144,17 +1160,16 @@ extern "C" dns_status dns_kernel_timing(dns_handle *h, const char **names, int64
     //   k_row     reads 4 fields M x H, writes 1: 5 * 0.75 N^2 * c           (60 N^2 | 30 N^2)
     //   k_col     reads G + om, fn, writes om, fn + V: 3.75 c N^2 + 32 N^2   (92 N^2 | 62 N^2)
     //   k_row_cfl reads u, w in FP64: 24 N^2
-    // multi-pass phases (per call; every line-FFT pass reads and writes its batch once):
-    //   big_spec   om in (8 N^2), 4 z-line fields out (3 c N^2)
-    //   big_zinv   2 passes over 4 fields: 12 c N^2      big_pack  3 c in, 4.5 c out
-    //   big_xinv   2 passes over 2 planes: 18 c N^2      big_prod  4.5 c in, 1.125 c out
-    //   big_xfwd   2 passes over 1/2 plane: 4.5 c N^2    big_unpack 1.125 c in, 0.75 c out
-    //   big_zfwd   2 passes over 1 field: 3 c N^2        big_cn    0.75 c in + 32 N^2 state
+    // multi-pass passes (state 16 B; c per intermediate element; M H = 0.75 N^2, M^2 = 2.25 N^2):
+    //   zinv_a  om once (8 N^2) in, 4 fields out (3 c)    zinv_b  4 fields in and out (6 c)
+    //   xinv_a  4 fields in (3 c), 2 planes out (4.5 c)   xinv_b  2 planes in, 1/2 plane out (5.625 c)
+    //   xfwd_b  1/2 plane in, 2/3 of it out (1.875 c)     zfwd_a  kept modes in, 1 field out (1.5 c)
+    //   zfwd_b  1 field in (0.75 c), om, fn in and out (32 N^2)
     const double c = h->fp32 ? 8.0 : 16.0;
-    const double bytes[KT_COUNT] = {3.75 * c * N2, (3.75 * c + 32.0) * N2, 24.0 * N2,
-                                    (8.0 + 3.0 * c) * N2, 12.0 * c * N2, 7.5 * c * N2,
-                                    18.0 * c * N2, 5.625 * c * N2, 4.5 * c * N2,
-                                    1.875 * c * N2, 3.0 * c * N2, (0.75 * c + 32.0) * N2};
+    const double bytes[KT_COUNT] = {3.75 * c * N2,        (3.75 * c + 32.0) * N2, 24.0 * N2,
+                                    (8.0 + 3.0 * c) * N2, 6.0 * c * N2,           7.5 * c * N2,
+                                    5.625 * c * N2,       1.875 * c * N2,         1.5 * c * N2,
+                                    (0.75 * c + 32.0) * N2};
     const int cnt = std::min(*n, (int)KT_COUNT);
     for (int i = 0; i < cnt; ++i) {
         if (names) names[i] = kKernelNames[i];

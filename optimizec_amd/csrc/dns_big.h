@@ -1,5 +1,13 @@
 // dns_big.h -- launch interface of the multi-pass time step (dns_big.hip) for grids
 // whose lines do not fit the fused LDS-resident kernels.  Internal to the library.
+//
+// Layouts (elements of the transform precision unless noted; hs, ms: padded row strides
+// of the H-wide and M-wide rows):
+//   om, fn  [iz][kx] FP64, N x H (unpadded: the host layout)
+//   Y, Z    [f][z][kx], 2 or 4 fields, M rows of stride hs
+//   X       [p][z][x1], 1 or 2 planes, M rows of stride ms
+//   Yq, Qh  [q][.], M/2 rows of stride ms
+//   W       [z1][kx], M rows of stride hs
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -10,20 +18,23 @@
 #include "dns_types.h"
 
 enum {
-    BIG_SPEC = 0,     // om -> nfields padded spectra as z-lines (a: out)
-    BIG_FFT_INV = 1,  // `lines` inverse line FFTs of length M: a -> b (scratch: pass-A output)
-    BIG_FFT_FWD = 2,  // forward, same arguments
-    BIG_PACK = 3,     // z-lines a (+ a2) -> Hermitian row spectra a + i a2 in rows b [M][M]
-    BIG_PROD = 4,     // rows a = u + i w, a2 = w_x + i w_z -> row-pair products b [M/2][M]
-    BIG_CFL = 5,      // rows a = u + i w -> cflrow[M]
-    BIG_UNPACK = 6,   // forward row-pair spectra a -> z-lines b [H][M]
-    BIG_CN = 7,       // z-forward lines a -> FN, CN/AB2 update of om, fn
-    BIG_PHYS = 8      // rows a -> out0 (Re) / out1 (Im), or |a| with out_mode 1 (FP64 out)
+    BIG_ZINV_A = 0,    // om -> Y (fields.pairs pairs of fields)                 b = Y
+    BIG_ZINV_B = 1,    // Y -> Z                                                 a = Y, b = Z
+    BIG_XINV_A = 2,    // Z -> X (planes 1: u + i w; 2: also w_x + i w_z)       a = Z, b = X
+    BIG_XINV_B = 3,    // X -> Yq (product, first forward step)                  a = X, b = Yq
+    BIG_XINV_CFL = 4,  // X (plane 0) -> cflrow[q * cfl_tiles + t] maxima        a = X
+    BIG_XFWD_B = 5,    // Yq -> Qh                                               a = Yq, b = Qh
+    BIG_ZFWD_A = 6,    // Qh -> W                                                a = Qh, b = W
+    BIG_ZFWD_B = 7,    // W -> FN, CN/AB2 on om, fn                              a = W
+    BIG_LINE_INV = 8,  // FP64: `lines` natural-order inverse line FFTs a -> b (scratch)
+    BIG_ROWS = 9,      // FP64: Z fields 0 (+ i 1 when planes == 2) -> rows R [z][x]  a = Z, b = R
+    BIG_ROWS_PHYS = 10,  // FP64: rows a -> out0 (Re, or |.| with out_mode 1)
+    BIG_ENERGY = 11    // om -> per-row E, Z partials out0[N], out1[N]
 };
 
-struct BigSpecArgs {
-    int nfields;
-    int fields[4];  // DNS_F_*
+struct BigFields {
+    int f[4];   // DNS_F_* of each field (pairs (f0, f1), (f2, f3))
+    int pairs;  // 1 or 2
 };
 
 struct BigCnArgs {
@@ -34,17 +45,18 @@ struct BigCnArgs {
 
 struct BigReq {
     int what = 0;
-    int fp32 = 0;   // element type of a, a2, b, scratch, tw: cf (1) or cd (0)
+    int fp32 = 0;   // element type of the intermediates and tw: cf (1) or cd (0)
     int H = 0;
     hipStream_t stream = nullptr;
     const void *tw = nullptr;  // W_M^t, t in [0, M)
     dnsfft::cd *om = nullptr, *fn = nullptr;
-    void *a = nullptr;
-    const void *a2 = nullptr;
+    const void *a = nullptr;
     void *b = nullptr;
     void *scratch = nullptr;
     size_t lines = 0;
-    BigSpecArgs spec{};
+    size_t hs = 0, ms = 0;
+    int planes = 2;
+    BigFields fields{};
     BigCnArgs cn{};
     double *cflrow = nullptr;
     double DX = 0.0, DZ = 0.0;
@@ -52,6 +64,8 @@ struct BigReq {
     int out_mode = 0;
 };
 
-// 1 if line length M = 3N/2 has a built four-step split
+// 1 if line length M = 3N/2 has a built split
 int dns_big_supported_m(int M);
+// CFL maxima per row pair written by BIG_XINV_CFL
+int dns_big_cfl_tiles(int M);
 hipError_t dns_big_launch(const BigReq &r);

@@ -1,22 +1,32 @@
-// dns_big.hip -- the time step for grids whose half-width H = N/2 is too long for the
-// fused LDS-resident row/column kernels (e.g. N = 8192, N = 29400: 44100^2 padded).
+// dns_big.hip -- the time step for grids whose lines do not fit the fused LDS-resident
+// kernels (N = 8192, 16384, 29400: 44100^2 padded), as seven passes through HBM.
 //
-// The state (om, fn: [kx][iz]) is the same as in the fused path; the transforms run
-// as batched M-point complex line FFTs through HBM, each split M = L1 x L2 into two
-// LDS-resident passes (four-step):
-//   pass A  (per line, C adjacent n2):  y[k1 L2 + n2] = W_M^{s n2 k1} FFT_L1(x[n1 L2 + n2])[k1]
-//   pass B  (per line, C adjacent k1):  X[k1 + L1 k2] = FFT_L2(y[k1 L2 + n2])[k2]
-// One step (reference order mem_cuda.cu:1851-1857; FN in advective form):
-//   spec    om -> 4 padded spectra (u, w, d_x omega, d_z omega) as z-lines  [f][kx][z]
-//   z-inv   4H lines of M
-//   pack    (u, w) and (d_x omega, d_z omega) -> full x-spectra a + i b of each row
-//           (Hermitian extension; LDS-tiled transpose)                     [p][z][x]
-//   x-inv   2M lines of M  -> (u + i w), (w_x + i w_z) on the M x M grid
-//   prod    P = u w_x + w w_z, two rows packed as one complex line          [M/2][x]
-//   x-fwd   M/2 lines of M
-//   unpack  split the row pair, keep kx < H, transpose to z-lines          [kx][z]
-//   z-fwd   H lines of M
-//   cn      FN = -P^/M^2 (kz = -N/2 row 0), CN/AB2 update of om, fn
+// Every M-point transform is a four-step pair of LDS passes, M = L1 x L2:
+//   pass A (L1-point FFTs over n1 of the points n1 L2 + n2, times W_M^{-/+ n2 k1})
+//   pass B (L2-point FFTs over n2 of the pass-A results of one k1 -> point k1 + L1 k2)
+// The state om, fn is [iz][kx] (rows of H = N/2 modes, the host layout), and every
+// intermediate is row-major with kx (or x) contiguous, so the z-direction passes read
+// and write runs of BZ adjacent kx and no transpose is needed anywhere.  One step
+// (reference order mem_cuda.cu:1851-1857; advective form of FN):
+//   1 zinv_a   om -> spectra of (u, w) and (d_x omega, d_z omega) on the padded z points,
+//              z pass A                                                   Y [f][z1][kx]
+//   2 zinv_b   z pass B -> the 4 fields on the physical rows z             Z [f][z][kx]
+//   3 xinv_a   Hermitian row spectra u + i w and w_x + i w_z built on the fly from Z,
+//              x pass A                                                    X [p][z][x1]
+//   4 xinv_b   x pass B of rows 2q, 2q+1 (both pairs) -> physical values, product
+//              P = u w_x + w w_z, the two rows packed as P_2q + i P_2q+1, and the first
+//              forward step (L2-point FFT over k2 of each k1 line, x W_M^{k1 m2})  Yq [q][k1][m2]
+//   5 xfwd_b   L1-point FFTs over k1 -> the packed rows' spectra, kept for kx < H and
+//              M - kx                                                     Qh [q][kx]
+//   6 zfwd_a   the two rows' spectra split apart (F_2q, F_2q+1) while loading, z pass A
+//                                                                          W [z1][kx]
+//   7 zfwd_b   z pass B, FN = -X / M^2 on the retained kz (kz = -N/2 row 0), CN/AB2
+//              update of om, fn (step3_cuda.cu:75-100)
+// The physical x order after pass 4 is (k1, k2) instead of k1 + L1 k2; the forward
+// transform of passes 4-5 is written for that order, so nothing is reordered.
+//
+// Diagnostics (fields, CFL) reuse passes 1-2 in FP64, build natural-order rows and run
+// plain batched four-step line FFTs (k_line_a / k_line_b).
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -29,65 +39,36 @@ using namespace dnsfft;
 
 namespace {
 
+constexpr int kT = 256;
+
 __device__ __forceinline__ double gam(int iz, int N) { return (iz < (N >> 1)) ? double(iz) : double(iz - N); }
 
-// ---------------------------------------------------------------------------------
-// batched four-step line FFT, pass A
-// ---------------------------------------------------------------------------------
-template <class TC, int L1, int L2, int C, int T, bool INV>
-__global__ __launch_bounds__(T) void k_big_fft_a(const TC *__restrict__ x, TC *__restrict__ y,
-                                                 const TC *__restrict__ tw)
-{
-    constexpr int L = L1 * L2;
-    constexpr int LS = padded_len(L1);
-    __shared__ TC lds[C * LS];
-    const size_t line = blockIdx.y;
-    const int n20 = blockIdx.x * C;
-    const TC *src = x + line * L;
-    TC *dst = y + line * L;
-    for (int e = threadIdx.x; e < C * L1; e += T) {
-        const int c = e % C, n1 = e / C;
-        lds[c * LS + pidx(n1)] = src[(size_t)n1 * L2 + n20 + c];
-    }
-    __syncthreads();
-    fft_lines<L1, LS, C, T, INV>(lds, tw, L2);
-    for (int e = threadIdx.x; e < C * L1; e += T) {
-        const int c = e % C, k1 = e / C;
-        const int n2 = n20 + c;
-        const TC w = tw[opaque((int)(((long long)n2 * k1) % L))];
-        const TC v = lds[c * LS + pidx(k1)];
-        dst[(size_t)k1 * L2 + n2] = INV ? mulc(v, w) : mul(v, w);
-    }
-}
+// split M = L1 x L2 and the pass tiles: C3 | L2 (pass 3), C4 | L1 (pass 4), C5 | L2 (pass 5)
+template <int M_>
+struct Split;
+#define BIG_SPLIT(M_, L1_, L2_, C3_, C4_, C5_)                                                                  \
+    template <>                                                                                                 \
+    struct Split<M_> {                                                                                          \
+        static constexpr int L1 = L1_, L2 = L2_, C3 = C3_, C4 = C4_, C5 = C5_;                                  \
+        static_assert(L1 * L2 == M_ && L2 % 2 == 0 && L2 % C3 == 0 && L1 % C4 == 0 && L2 % C5 == 0, "split"); \
+    };
+BIG_SPLIT(96, 8, 12, 4, 4, 4)
+BIG_SPLIT(192, 12, 16, 8, 4, 8)
+BIG_SPLIT(384, 16, 24, 8, 4, 8)
+BIG_SPLIT(768, 24, 32, 8, 4, 8)
+BIG_SPLIT(1536, 32, 48, 8, 4, 8)
+BIG_SPLIT(6144, 64, 96, 8, 4, 8)
+BIG_SPLIT(12288, 96, 128, 8, 4, 8)
+BIG_SPLIT(24576, 128, 192, 8, 4, 8)
+BIG_SPLIT(44100, 210, 210, 6, 3, 14)
+#undef BIG_SPLIT
 
-// ---------------------------------------------------------------------------------
-// batched four-step line FFT, pass B (natural-order output)
-// ---------------------------------------------------------------------------------
-template <class TC, int L1, int L2, int C, int T, bool INV>
-__global__ __launch_bounds__(T) void k_big_fft_b(const TC *__restrict__ y, TC *__restrict__ out,
-                                                 const TC *__restrict__ tw)
-{
-    constexpr int L = L1 * L2;
-    constexpr int LS = padded_len(L2);
-    __shared__ TC lds[C * LS];
-    const size_t line = blockIdx.y;
-    const int k10 = blockIdx.x * C;
-    const TC *src = y + line * L;
-    TC *dst = out + line * L;
-    for (int e = threadIdx.x; e < C * L2; e += T) {
-        const int c = e / L2, n2 = e % L2;
-        lds[c * LS + pidx(n2)] = src[(size_t)(k10 + c) * L2 + n2];
-    }
-    __syncthreads();
-    fft_lines<L2, LS, C, T, INV>(lds, tw, L1);
-    for (int e = threadIdx.x; e < C * L2; e += T) {
-        const int c = e % C, k2 = e / C;
-        dst[(size_t)k2 * L1 + k10 + c] = lds[c * LS + pidx(k2)];
-    }
-}
+// adjacent kx per z-direction workgroup: 128 bytes of elements
+template <class TC>
+constexpr int bz_of() { return sizeof(TC) == 16 ? 8 : 16; }
 
-// spectrum multiplier of field f (the fused column kernel's conventions: the
-// multipliers of omega listed with DNS_F_*)
+// spectrum multiplier of field f on omega-hat (the fused column kernel's conventions:
+// the multipliers listed with DNS_F_*)
 __device__ __forceinline__ cd spec_of(int f, cd om, int ix, int iz, int N)
 {
     const double ax = double(ix), gz = gam(iz, N);
@@ -111,304 +92,627 @@ __device__ __forceinline__ cd spec_of(int f, cd om, int ix, int iz, int N)
     return ti ? cd{-s * om.y, s * om.x} : cd{s * om.x, s * om.y};
 }
 
-// om [kx][iz] -> padded z-lines Z[fi][kx][z'] (z' in [0, M); the band [H, 2H) is zero)
-template <class TC>
-__global__ void k_big_spec(const cd *__restrict__ om, TC *__restrict__ Z, BigSpecArgs a, int H)
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Kernels
+// launch a 1-D grid and take logical block xcd_order(b): consecutive logical blocks --
+// neighbouring tiles, which share 128-byte lines at their edges -- run on one XCD.
+__device__ __forceinline__ int xcd_order(int b, int n) { return (n % 8 == 0) ? (b % 8) * (n / 8) + b / 8 : b; }
+
+// W_M^{e} from the table, conjugated for the inverse direction
+template <class TC, bool INV>
+__device__ __forceinline__ TC twid(const TC *__restrict__ tw, int e)
 {
-    const int N = 2 * H, M = 3 * H;
-    const size_t total = (size_t)H * M;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int kx = (int)(e / M), zp = (int)(e % M);
-        int iz = -1;
-        if (zp < H)
-            iz = zp;
-        else if (zp >= 2 * H)
-            iz = zp - H;
-        const cd o = (iz >= 0) ? om[(size_t)kx * N + iz] : cd{0.0, 0.0};
-        for (int fi = 0; fi < a.nfields; ++fi) {
-            const cd v = (iz >= 0) ? spec_of(a.fields[fi], o, kx, iz, N) : cd{0.0, 0.0};
-            Z[(size_t)fi * total + e] = from_d<TC>(v);
+    const TC w = tw[opaque(e)];
+    return INV ? conj(w) : w;
+}
+
+// ---- pass 1: om -> two field spectra on the padded z points, z pass A ------------------
+template <class TC, int M_>
+__global__ __launch_bounds__(kT) void k_zinv_a(const cd *__restrict__ om, TC *__restrict__ Y, const TC *__restrict__ tw,
+                                               BigFields fl, int H, size_t hs)
+{
+    using S = Split<M_>;
+    constexpr int L1 = S::L1, L2 = S::L2, BZ = bz_of<TC>(), LS = padded_len(L1);
+    __shared__ TC lds[2 * BZ * LS];
+    const int N = 2 * H;
+    const int tiles = (H + BZ - 1) / BZ;
+    const int lb = xcd_order(blockIdx.x, gridDim.x);
+    const int kx0 = (lb % tiles) * BZ, n2 = (lb / tiles) % L2, pr = lb / (tiles * L2);
+    const int fa = fl.f[2 * pr], fb = fl.f[2 * pr + 1];
+    // all of a thread's loads first (independent, in flight together), then the LDS writes
+    constexpr int CNT = BZ * L1, IT = (CNT + kT - 1) / kT;
+    cd ov[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        const int c = e % BZ, n1 = e / BZ;
+        const int kx = kx0 + c, zp = n1 * L2 + n2;
+        const int iz = zp < H ? zp : (zp >= 2 * H ? zp - H : -1);
+        ov[i] = (e < CNT && kx < H && iz >= 0) ? om[(size_t)iz * H + kx] : cd{0.0, 0.0};
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        if (e < CNT) {
+            const int c = e % BZ, n1 = e / BZ;
+            const int kx = kx0 + c, zp = n1 * L2 + n2;
+            const int iz = zp < H ? zp : (zp >= 2 * H ? zp - H : -1);
+            TC va{0, 0}, vb{0, 0};
+            if (kx < H && iz >= 0) {
+                va = from_d<TC>(spec_of(fa, ov[i], kx, iz, N));
+                vb = from_d<TC>(spec_of(fb, ov[i], kx, iz, N));
+            }
+            lds[c * LS + pidx(n1)] = va;
+            lds[(BZ + c) * LS + pidx(n1)] = vb;
+        }
+    }
+    __syncthreads();
+    fft_lines<L1, LS, 2 * BZ, kT, true>(lds, tw, L2);
+    TC *ya = Y + (size_t)(2 * pr) * M_ * hs, *yb = ya + (size_t)M_ * hs;
+    for (int e = threadIdx.x; e < BZ * L1; e += kT) {
+        const int c = e % BZ, k1 = e / BZ;
+        const int kx = kx0 + c;
+        if (kx < H) {
+            const TC w = twid<TC, true>(tw, n2 * k1);
+            const size_t o = (size_t)(k1 * L2 + n2) * hs + kx;
+            ya[o] = mul(lds[c * LS + pidx(k1)], w);
+            yb[o] = mul(lds[(BZ + c) * LS + pidx(k1)], w);
         }
     }
 }
 
-// Row x-spectra of two fields a, b (z-lines, [kx][z]) packed as a + i b with the
-// Hermitian extension, transposed through a 32 x 32 LDS tile into rows R[z][x].
-template <class TC>
-__global__ __launch_bounds__(256) void k_big_pack(const TC *__restrict__ Za, const TC *__restrict__ Zb,
-                                                  TC *__restrict__ R, int H)
+// ---- pass 2: z pass B -> fields on the physical rows ------------------------------------
+template <class TC, int M_>
+__global__ __launch_bounds__(kT) void k_zinv_b(const TC *__restrict__ Y, TC *__restrict__ Z, const TC *__restrict__ tw,
+                                               int H, size_t hs)
 {
-    using RT = decltype(TC::x);
-    const int M = 3 * H;
-    __shared__ TC lo[32][33];
-    __shared__ TC hi[32][33];
-    const int kx0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-    for (int r = ty; r < 32; r += 8) {
-        const int kx = kx0 + r, z = z0 + tx;
-        TC zl{RT(0), RT(0)}, zh{RT(0), RT(0)};
-        if (kx < H && z < M) {
-            const TC av = Za[(size_t)kx * M + z];
-            const TC bv = Zb ? Zb[(size_t)kx * M + z] : TC{RT(0), RT(0)};
-            if (kx == 0) {
-                zl = TC{av.x, bv.x};
-            } else {
-                zl = TC{av.x - bv.y, av.y + bv.x};
-                zh = TC{av.x + bv.y, bv.x - av.y};  // conj(a) + i conj(b)
+    using S = Split<M_>;
+    constexpr int L1 = S::L1, L2 = S::L2, BZ = bz_of<TC>(), LS = padded_len(L2);
+    __shared__ TC lds[2 * BZ * LS];
+    const int tiles = (H + BZ - 1) / BZ;
+    const int lb = xcd_order(blockIdx.x, gridDim.x);
+    const int kx0 = (lb % tiles) * BZ, k1 = (lb / tiles) % L1, pr = lb / (tiles * L1);
+    const TC *ya = Y + (size_t)(2 * pr) * M_ * hs, *yb = ya + (size_t)M_ * hs;
+    constexpr int CNT = BZ * L2, IT = (CNT + kT - 1) / kT;
+    TC va[IT], vb[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        const int c = e % BZ, n2 = e / BZ;
+        const int kx = kx0 + c;
+        const size_t o = (size_t)(k1 * L2 + n2) * hs + kx;
+        const bool ok = e < CNT && kx < H;
+        va[i] = ok ? ya[o] : TC{0, 0};
+        vb[i] = ok ? yb[o] : TC{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        if (e < CNT) {
+            const int c = e % BZ, n2 = e / BZ;
+            lds[c * LS + pidx(n2)] = va[i];
+            lds[(BZ + c) * LS + pidx(n2)] = vb[i];
+        }
+    }
+    __syncthreads();
+    fft_lines<L2, LS, 2 * BZ, kT, true>(lds, tw, L1);
+    TC *za = Z + (size_t)(2 * pr) * M_ * hs, *zb = za + (size_t)M_ * hs;
+    for (int e = threadIdx.x; e < BZ * L2; e += kT) {
+        const int c = e % BZ, k2 = e / BZ;
+        const int kx = kx0 + c;
+        if (kx < H) {
+            const size_t o = (size_t)(k1 + L1 * k2) * hs + kx;
+            za[o] = lds[c * LS + pidx(k2)];
+            zb[o] = lds[(BZ + c) * LS + pidx(k2)];
+        }
+    }
+}
+
+// Hermitian row spectrum a + i b at kx from two fields' row (kx < H retained; bin 0 keeps
+// the real parts -- the C2R convention; M - kx mirrors; the band between is zero)
+template <class TC>
+__device__ __forceinline__ TC row_spec(const TC *__restrict__ a, const TC *__restrict__ b, int kx, int H, int M)
+{
+    if (kx == 0) return TC{a[0].x, b[0].x};
+    if (kx < H) {
+        const TC av = a[kx], bv = b[kx];
+        return TC{av.x - bv.y, av.y + bv.x};
+    }
+    if (kx > M - H) {
+        const TC av = a[M - kx], bv = b[M - kx];
+        return TC{av.x + bv.y, bv.x - av.y};
+    }
+    return TC{0, 0};
+}
+
+// ---- pass 3: row spectra built from Z, x pass A ------------------------------------------
+// A workgroup owns a mirror-closed set of columns n2 of one row and one plane p (p = 0:
+// u + i w from fields 0, 1; p = 1: w_x + i w_z from fields 2, 3): the pairs
+// {j, L2 - j} for CP consecutive j in [1, L2/2), or (the last tile) the self-mirrored
+// columns 0 and L2/2.  Positions kx and M - kx then both lie in its lines, so each
+// retained value of the row is read once and fills its direct slot (a + i b) and its
+// mirror slot (conj a + i conj b); the band H <= kx <= M - H is zero.
+template <class TC, int M_>
+__global__ __launch_bounds__(kT) void k_xinv_a(const TC *__restrict__ Z, TC *__restrict__ X, const TC *__restrict__ tw,
+                                               int H, size_t hs, size_t ms)
+{
+    using S = Split<M_>;
+    constexpr int L1 = S::L1, L2 = S::L2, CP = bz_of<TC>(), LS = padded_len(L1);
+    constexpr int NPT = (L2 / 2 - 1 + CP - 1) / CP;  // pair tiles; tile NPT holds columns 0, L2/2
+    __shared__ TC lds[2 * CP * LS];
+    const int lb = xcd_order(blockIdx.x, gridDim.x);
+    const int t = lb % (NPT + 1), z = (lb / (NPT + 1)) % M_, p = lb / ((NPT + 1) * M_);
+    const TC *a = Z + ((size_t)(2 * p) * M_ + z) * hs;
+    const TC *b = a + (size_t)M_ * hs;
+    constexpr int CNT = CP * L1, IT = (CNT + kT - 1) / kT;
+    TC va[IT], vb[IT];
+    if (t < NPT) {
+        // slot pair (line i, n1) = position kp = n1 L2 + d, d = 1 + t CP + i, and (line CP + i,
+        // L1 - 1 - n1) = position M - kp; the smaller of the two is the retained one
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const int e = threadIdx.x + k * kT;
+            const int i = e % CP, n1 = e / CP;
+            const int d = 1 + t * CP + i;
+            const int kp = n1 * L2 + d, lo = kp < M_ - kp ? kp : M_ - kp;
+            const bool ok = e < CNT && d < L2 / 2 && lo < H;
+            va[k] = ok ? a[lo] : TC{0, 0};
+            vb[k] = ok ? b[lo] : TC{0, 0};
+        }
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const int e = threadIdx.x + k * kT;
+            if (e < CNT) {
+                const int i = e % CP, n1 = e / CP;
+                const int kp = n1 * L2 + 1 + t * CP + i;
+                const TC av = va[k], bv = vb[k];
+                const TC dir{av.x - bv.y, av.y + bv.x}, mir{av.x + bv.y, bv.x - av.y};
+                const bool first = kp < M_ - kp;  // this slot holds the retained position
+                lds[i * LS + pidx(n1)] = first ? dir : mir;
+                lds[(CP + i) * LS + pidx(L1 - 1 - n1)] = first ? mir : dir;
             }
         }
-        lo[r][tx] = zl;
-        hi[r][tx] = zh;
+    } else {
+        // columns 0 and L2/2 (lines 0, 1), each its own mirror; other lines unused
+        constexpr int CNT2 = 2 * L1, IT2 = (CNT2 + kT - 1) / kT;
+#pragma unroll
+        for (int k = 0; k < IT2; ++k) {
+            const int e = threadIdx.x + k * kT;
+            const int col = (e / L1) * (L2 / 2), n1 = e % L1;
+            const int kp = n1 * L2 + col;
+            const int src = kp < H ? kp : (kp > M_ - H ? M_ - kp : -1);
+            TC v{0, 0};
+            if (e < CNT2 && src >= 0) {
+                const TC av = a[src], bv = b[src];
+                if (kp == 0)
+                    v = TC{av.x, bv.x};
+                else if (kp < H)
+                    v = TC{av.x - bv.y, av.y + bv.x};
+                else
+                    v = TC{av.x + bv.y, bv.x - av.y};
+            }
+            if (e < CNT2) lds[(e / L1) * LS + pidx(n1)] = v;
+        }
+        for (int e = threadIdx.x; e < (2 * CP - 2) * L1; e += kT) lds[(2 + e / L1) * LS + pidx(e % L1)] = TC{0, 0};
     }
     __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-        const int z = z0 + r, kx = kx0 + tx;
-        if (z < M && kx < H) {
-            TC *row = R + (size_t)z * M;
-            row[kx] = lo[tx][r];
-            if (kx >= 1) row[M - kx] = hi[tx][r];
+    fft_lines<L1, LS, 2 * CP, kT, true>(lds, tw, L2);
+    const int nl = t < NPT ? 2 * CP : 2;
+    for (int e = threadIdx.x; e < nl * L1; e += kT) {
+        const int l = e % nl, k1 = e / nl;
+        int n2;
+        if (t < NPT) {
+            const int d = 1 + t * CP + (l % CP);
+            if (d >= L2 / 2) continue;
+            n2 = l < CP ? d : L2 - d;
+        } else {
+            n2 = l * (L2 / 2);
+        }
+        X[((size_t)p * M_ + z) * ms + (size_t)k1 * L2 + n2] = mul(lds[l * LS + pidx(k1)], twid<TC, true>(tw, n2 * k1));
+    }
+}
+
+// ---- pass 4: x pass B of a row pair, product, first forward step -------------------------
+// MODE 0: product -> Yq;  MODE 1: CFL maxima of u, w (plane 0 only) -> cflpart
+template <class TC, int M_, int MODE>
+__global__ __launch_bounds__(kT) void k_xinv_b(const TC *__restrict__ X, TC *__restrict__ Yq, const TC *__restrict__ tw,
+                                               size_t ms, double *__restrict__ cflpart, double DX, double DZ)
+{
+    using S = Split<M_>;
+    constexpr int L1 = S::L1, L2 = S::L2, C = S::C4, LS = padded_len(L2);
+    constexpr int G = MODE == 0 ? 4 : 2;  // lines per k1: group g = 2 p + r (plane p, row 2q + r)
+    __shared__ TC lds[G * C * LS];
+    __shared__ double red[kT / 64];
+    using RT = decltype(TC::x);
+    const int q = blockIdx.x, k10 = blockIdx.y * C;
+    constexpr int CNT = G * C * L2, IT = (CNT + kT - 1) / kT;
+    TC xv[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        const int n2 = e % L2, rest = e / L2, c = rest % C, g = rest / C;
+        const int p = g >> 1, r = g & 1;
+        xv[i] = e < CNT ? X[((size_t)p * M_ + 2 * q + r) * ms + (size_t)(k10 + c) * L2 + n2] : TC{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        if (e < CNT) {
+            const int n2 = e % L2, rest = e / L2, c = rest % C, g = rest / C;
+            lds[(g * C + c) * LS + pidx(n2)] = xv[i];
         }
     }
-}
-
-// zero the band kx in [H, M - H] of every row (the mirror fills (M - H, M))
-template <class TC>
-__global__ void k_big_zero_band(TC *__restrict__ R, int H, int rows)
-{
-    using RT = decltype(TC::x);
-    const int M = 3 * H, w = M - 2 * H + 1;
-    const size_t total = (size_t)rows * w;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const size_t z = e / w;
-        R[z * M + H + (e % w)] = TC{RT(0), RT(0)};
-    }
-}
-
-// P = u w_x + w w_z from RA = u + i w, RB = w_x + i w_z; rows 2q, 2q+1 packed into Q[q];
-// optional per-row-pair CFL maxima
-template <class TC>
-__global__ __launch_bounds__(256) void k_big_prod(const TC *__restrict__ RA, const TC *__restrict__ RB,
-                                                  TC *__restrict__ Q, int M, double *__restrict__ cflrow,
-                                                  double DX, double DZ)
-{
-    __shared__ double red[4];
-    const int q = blockIdx.x;
-    double cm = 0.0;
-    for (int x = threadIdx.x; x < M; x += 256) {
-        const TC a0 = RA[(size_t)(2 * q) * M + x], b0 = RB[(size_t)(2 * q) * M + x];
-        const TC a1 = RA[(size_t)(2 * q + 1) * M + x], b1 = RB[(size_t)(2 * q + 1) * M + x];
-        Q[(size_t)q * M + x] = TC{a0.x * b0.x + a0.y * b0.y, a1.x * b1.x + a1.y * b1.y};
-        const double c0 = fabs(double(a0.x)) / DX + fabs(double(a0.y)) / DZ;
-        const double c1 = fabs(double(a1.x)) / DX + fabs(double(a1.y)) / DZ;
-        cm = fmax(cm, fmax(c0, c1));
-    }
-    if (cflrow) {
-        for (int off = 32; off > 0; off >>= 1) cm = fmax(cm, __shfl_xor(cm, off, 64));
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cm;
+    __syncthreads();
+    fft_lines<L2, LS, G * C, kT, true>(lds, tw, L1);
+    if (MODE == 1) {
+        double m = 0.0;
+        for (int e = threadIdx.x; e < 2 * C * L2; e += kT) {
+            const TC v = lds[(e / L2) * LS + pidx(e % L2)];
+            const double cfl = fabs(double(v.x)) / DX + fabs(double(v.y)) / DZ;
+            m = cfl > m ? cfl : m;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(m, off, 64);
+            m = o > m ? o : m;
+        }
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
         __syncthreads();
-        if (threadIdx.x == 0) cflrow[q] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        if (threadIdx.x == 0) {
+            double t = red[0];
+            for (int w = 1; w < kT / 64; ++w) t = red[w] > t ? red[w] : t;
+            cflpart[(size_t)q * gridDim.y + blockIdx.y] = t;
+        }
+        return;
     }
-}
-
-// per-row CFL maxima of R = u + i w
-template <class TC>
-__global__ __launch_bounds__(256) void k_big_cfl(const TC *__restrict__ R, int M, double *__restrict__ cflrow,
-                                                 double DX, double DZ)
-{
-    __shared__ double red[4];
-    const int z = blockIdx.x;
-    double cm = 0.0;
-    for (int x = threadIdx.x; x < M; x += 256) {
-        const TC v = R[(size_t)z * M + x];
-        cm = fmax(cm, fabs(double(v.x)) / DX + fabs(double(v.y)) / DZ);
+    // P = u w_x + w w_z on rows 2q (real part) and 2q + 1 (imaginary part): group 0/1 hold
+    // u + i w of the two rows, group 2/3 w_x + i w_z
+    for (int e = threadIdx.x; e < C * L2; e += kT) {
+        const int c = e / L2, k2 = e % L2;
+        const TC a0 = lds[(0 * C + c) * LS + pidx(k2)], a1 = lds[(1 * C + c) * LS + pidx(k2)];
+        const TC b0 = lds[(2 * C + c) * LS + pidx(k2)], b1 = lds[(3 * C + c) * LS + pidx(k2)];
+        lds[c * LS + pidx(k2)] = TC{RT(a0.x * b0.x + a0.y * b0.y), RT(a1.x * b1.x + a1.y * b1.y)};
     }
-    for (int off = 32; off > 0; off >>= 1) cm = fmax(cm, __shfl_xor(cm, off, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cm;
     __syncthreads();
-    if (threadIdx.x == 0) cflrow[z] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    fft_lines<L2, LS, C, kT, false>(lds, tw, L1);
+    for (int e = threadIdx.x; e < C * L2; e += kT) {
+        const int c = e / L2, m2 = e % L2;
+        const int k1 = k10 + c;
+        Yq[(size_t)q * ms + (size_t)k1 * L2 + m2] = mul(lds[c * LS + pidx(m2)], twid<TC, false>(tw, k1 * m2));
+    }
 }
 
-// Q^ (row pairs) -> the two rows' forward spectra for kx < H, transposed into z-lines
-// G[kx][z]: F_a = (Z[k] + conj Z[M-k]) / 2, F_b = (Z[k] - conj Z[M-k]) / 2i
-template <class TC>
-__global__ __launch_bounds__(256) void k_big_unpack(const TC *__restrict__ Q, TC *__restrict__ G, int H)
+// ---- pass 5: L1-point FFTs over k1 -> packed row-pair spectra (kept modes only) ----------
+template <class TC, int M_>
+__global__ __launch_bounds__(kT) void k_xfwd_b(const TC *__restrict__ Yq, TC *__restrict__ Qh, const TC *__restrict__ tw,
+                                               int H, size_t ms)
 {
+    using S = Split<M_>;
+    constexpr int L1 = S::L1, L2 = S::L2, C = S::C5, LS = padded_len(L1);
+    __shared__ TC lds[C * LS];
+    const int lb = xcd_order(blockIdx.x, gridDim.x);
+    const int m20 = (lb % (L2 / C)) * C, q = lb / (L2 / C);
+    constexpr int CNT = C * L1, IT = (CNT + kT - 1) / kT;
+    TC yv[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        const int c = e % C, k1 = e / C;
+        yv[i] = e < CNT ? Yq[(size_t)q * ms + (size_t)k1 * L2 + m20 + c] : TC{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        if (e < CNT) lds[(e % C) * LS + pidx(e / C)] = yv[i];
+    }
+    __syncthreads();
+    fft_lines<L1, LS, C, kT, false>(lds, tw, L2);
+    for (int e = threadIdx.x; e < C * L1; e += kT) {
+        const int c = e % C, m1 = e / C;
+        const int kx = m20 + c + L2 * m1;
+        if (kx < H || kx > M_ - H) Qh[(size_t)q * ms + kx] = lds[c * LS + pidx(m1)];
+    }
+}
+
+// ---- pass 6: split the row pairs, z pass A (forward) -------------------------------------
+template <class TC, int M_>
+__global__ __launch_bounds__(kT) void k_zfwd_a(const TC *__restrict__ Qh, TC *__restrict__ W, const TC *__restrict__ tw,
+                                               int H, size_t hs, size_t ms)
+{
+    using S = Split<M_>;
+    constexpr int L1 = S::L1, L2 = S::L2, BZ = bz_of<TC>(), LS = padded_len(L1);
+    __shared__ TC lds[2 * BZ * LS];
     using RT = decltype(TC::x);
-    const int M = 3 * H;
-    __shared__ TC ta[32][33];
-    __shared__ TC tb[32][33];
-    const int kx0 = blockIdx.x * 32, q0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int tiles = (H + BZ - 1) / BZ;
+    const int lb = xcd_order(blockIdx.x, gridDim.x);
+    const int kx0 = (lb % tiles) * BZ, j = lb / tiles;  // n2 = 2 j + r: rows z of parity r
     const RT half = RT(0.5);
-    for (int r = ty; r < 32; r += 8) {
-        const int q = q0 + r, kx = kx0 + tx;
-        TC fa{RT(0), RT(0)}, fb{RT(0), RT(0)};
-        if (q < M / 2 && kx < H) {
-            const TC zk = Q[(size_t)q * M + kx];
-            const TC zm = conj(Q[(size_t)q * M + (kx == 0 ? 0 : M - kx)]);
-            const TC s = add(zk, zm), d = sub(zk, zm);
-            fa = TC{half * s.x, half * s.y};
-            fb = TC{half * d.y, -half * d.x};
+    constexpr int CNT = BZ * L1, IT = (CNT + kT - 1) / kT;
+    TC zkv[IT], zmv[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        const int c = e % BZ, n1 = e / BZ;
+        const int kx = kx0 + c;
+        const size_t q = (size_t)(n1 * L2 + 2 * j) / 2;
+        const bool ok = e < CNT && kx < H;
+        zkv[i] = ok ? Qh[q * ms + kx] : TC{0, 0};
+        zmv[i] = ok ? Qh[q * ms + (kx == 0 ? 0 : M_ - kx)] : TC{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        if (e < CNT) {
+            const int c = e % BZ, n1 = e / BZ;
+            const TC zk = zkv[i], zm = conj(zmv[i]);
+            const TC sm = add(zk, zm), df = sub(zk, zm);
+            lds[c * LS + pidx(n1)] = TC{half * sm.x, half * sm.y};
+            lds[(BZ + c) * LS + pidx(n1)] = TC{half * df.y, -half * df.x};
         }
-        ta[r][tx] = fa;
-        tb[r][tx] = fb;
     }
     __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-        const int kx = kx0 + r, q = q0 + tx;
-        if (kx < H && q < M / 2) {
-            G[(size_t)kx * M + 2 * q] = ta[tx][r];
-            G[(size_t)kx * M + 2 * q + 1] = tb[tx][r];
-        }
+    fft_lines<L1, LS, 2 * BZ, kT, false>(lds, tw, L2);
+    for (int e = threadIdx.x; e < 2 * BZ * L1; e += kT) {
+        const int c = e % BZ, rest = e / BZ, k1 = rest % L1, r = rest / L1;
+        const int kx = kx0 + c, n2 = 2 * j + r;
+        if (kx < H)
+            W[(size_t)(k1 * L2 + n2) * hs + kx] = mul(lds[(r * BZ + c) * LS + pidx(k1)], twid<TC, false>(tw, n2 * k1));
     }
 }
 
-// FN from the z-forward lines and the CN/AB2 update (step3_cuda.cu:75-100)
-template <class TC>
-__global__ void k_big_cn(cd *__restrict__ om, cd *__restrict__ fn, const TC *__restrict__ G, BigCnArgs a, int H)
+// ---- pass 7: z pass B (forward), FN, CN/AB2 ----------------------------------------------
+template <class TC, int M_>
+__global__ __launch_bounds__(kT) void k_zfwd_b(const TC *__restrict__ W, cd *__restrict__ om, cd *__restrict__ fn,
+                                               const TC *__restrict__ tw, int H, size_t hs, BigCnArgs a)
 {
-    const int N = 2 * H, M = 3 * H;
-    const size_t total = (size_t)H * N;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int kx = (int)(e / N), iz = (int)(e % N);
-        const int zp = (iz < H) ? iz : iz + H;
-        const cd X = to_d(G[(size_t)kx * M + zp]);
+    using S = Split<M_>;
+    constexpr int L1 = S::L1, L2 = S::L2, BZ = bz_of<TC>(), LS = padded_len(L2);
+    __shared__ TC lds[BZ * LS];
+    const int N = 2 * H;
+    const int tiles = (H + BZ - 1) / BZ;
+    const int lb = xcd_order(blockIdx.x, gridDim.x);
+    const int kx0 = (lb % tiles) * BZ, k1 = lb / tiles;
+    constexpr int CNT = BZ * L2, IT = (CNT + kT - 1) / kT;
+    TC wv[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        const int c = e % BZ, n2 = e / BZ;
+        const int kx = kx0 + c;
+        wv[i] = (e < CNT && kx < H) ? W[(size_t)(k1 * L2 + n2) * hs + kx] : TC{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = threadIdx.x + i * kT;
+        if (e < CNT) lds[(e % BZ) * LS + pidx(e / BZ)] = wv[i];
+    }
+    __syncthreads();
+    fft_lines<L2, LS, BZ, kT, false>(lds, tw, L1);
+    for (int e = threadIdx.x; e < BZ * L2; e += kT) {
+        const int c = e % BZ, k2 = e / BZ;
+        const int kx = kx0 + c, kz = k1 + L1 * k2;
+        const int iz = kz < H ? kz : (kz >= 2 * H ? kz - H : -1);
+        if (kx >= H || iz < 0) continue;
+        const cd X = to_d(lds[c * LS + pidx(k2)]);
         cd FN{-X.x * a.divxz, -X.y * a.divxz};
         if (iz == H) FN = cd{0.0, 0.0};
         const double ax = double(kx), gz = gam(iz, N);
         const double ARG = a.VT * (ax * ax + gz * gz);
         const double DEN = 1.0 + ARG, c1 = 1.0 - ARG;
-        const cd o = om[e], f = fn[e];
+        const size_t io = (size_t)iz * H + kx;
+        const cd o = om[io], f = fn[io];
         const cd num{c1 * o.x + a.c2 * FN.x + a.c3 * f.x, c1 * o.y + a.c2 * FN.y + a.c3 * f.y};
-        om[e] = cd{num.x / DEN, num.y / DEN};
-        fn[e] = FN;
+        om[io] = cd{num.x / DEN, num.y / DEN};
+        fn[io] = FN;
     }
 }
 
-// physical planes from R = a + i b (FP64): out0 = Re, out1 = Im (optional) or |R|
-template <class TC>
-__global__ void k_big_phys(const TC *__restrict__ R, size_t n, double *__restrict__ out0, double *__restrict__ out1,
-                           int out_mode)
+// ---- diagnostics: natural-order rows and plain batched line FFTs (FP64) -------------------
+// rows R[z][x] of the Hermitian spectrum of field 0 of Z (+ i field 1 when two)
+__global__ void k_rows(const cd *__restrict__ Z, cd *__restrict__ R, int H, int M, size_t hs, int two)
 {
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        const cd v = to_d(R[e]);
-        if (out_mode == 1) {
-            out0[e] = sqrt(v.x * v.x + v.y * v.y);
+    const size_t total = (size_t)M * M;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int z = (int)(e / M), x = (int)(e % M);
+        const cd *a = Z + (size_t)z * hs;
+        if (two) {
+            R[e] = row_spec(a, a + (size_t)M * hs, x, H, M);
         } else {
-            out0[e] = v.x;
-            if (out1) out1[e] = v.y;
+            cd v{0.0, 0.0};
+            if (x == 0)
+                v = cd{a[0].x, 0.0};
+            else if (x < H)
+                v = a[x];
+            else if (x > M - H)
+                v = conj(a[M - x]);
+            R[e] = v;
         }
     }
 }
 
-// ---------------------------------------------------------------------------------
-// line-length table: M = L1 x L2, C adjacent sub-lines per workgroup (C divides both)
-// ---------------------------------------------------------------------------------
-template <int M_>
-struct BigLen;
-#define BIG_LEN(M_, L1_, L2_, C_)          \
-    template <>                            \
-    struct BigLen<M_> {                    \
-        static constexpr int L1 = L1_;     \
-        static constexpr int L2 = L2_;     \
-        static constexpr int C = C_;       \
-    };
-BIG_LEN(96, 8, 12, 4)
-BIG_LEN(192, 12, 16, 4)
-BIG_LEN(384, 16, 24, 8)
-BIG_LEN(768, 24, 32, 8)
-BIG_LEN(6144, 64, 96, 16)
-BIG_LEN(1536, 32, 48, 16)
-BIG_LEN(12288, 96, 128, 16)
-BIG_LEN(24576, 128, 192, 16)
-BIG_LEN(44100, 210, 210, 14)
-#undef BIG_LEN
-
-template <class TC, int M_, bool INV>
-hipError_t line_fft_len(const TC *x, TC *y, TC *out, const TC *tw, size_t lines, hipStream_t s)
+template <int L1, int L2, int C, bool INV>
+__global__ __launch_bounds__(kT) void k_line_a(const cd *__restrict__ x, cd *__restrict__ y, const cd *__restrict__ tw)
 {
-    using B = BigLen<M_>;
-    static_assert(B::L1 * B::L2 == M_ && B::L1 % B::C == 0 && B::L2 % B::C == 0, "bad four-step split");
-    constexpr int T = 256;
-    hipLaunchKernelGGL((k_big_fft_a<TC, B::L1, B::L2, B::C, T, INV>), dim3(B::L2 / B::C, (unsigned)lines), dim3(T),
-                       0, s, x, y, tw);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_big_fft_b<TC, B::L1, B::L2, B::C, T, INV>), dim3(B::L1 / B::C, (unsigned)lines), dim3(T),
-                       0, s, y, out, tw);
-    return hipGetLastError();
+    constexpr int L = L1 * L2, LS = padded_len(L1);
+    __shared__ cd lds[C * LS];
+    const size_t line = blockIdx.y;
+    const int n20 = blockIdx.x * C;
+    const cd *src = x + line * L;
+    for (int e = threadIdx.x; e < C * L1; e += kT) {
+        const int c = e % C, n1 = e / C;
+        lds[c * LS + pidx(n1)] = src[(size_t)n1 * L2 + n20 + c];
+    }
+    __syncthreads();
+    fft_lines<L1, LS, C, kT, INV>(lds, tw, L2);
+    cd *dst = y + line * L;
+    for (int e = threadIdx.x; e < C * L1; e += kT) {
+        const int c = e % C, k1 = e / C;
+        const int n2 = n20 + c;
+        dst[(size_t)k1 * L2 + n2] = mul(lds[c * LS + pidx(k1)], twid<cd, INV>(tw, n2 * k1));
+    }
 }
 
-template <class TC, bool INV>
-hipError_t line_fft(int M, const TC *x, TC *y, TC *out, const TC *tw, size_t lines, hipStream_t s)
+template <int L1, int L2, int C, bool INV>
+__global__ __launch_bounds__(kT) void k_line_b(const cd *__restrict__ y, cd *__restrict__ out, const cd *__restrict__ tw)
 {
-    switch (M) {
-        case 96: return line_fft_len<TC, 96, INV>(x, y, out, tw, lines, s);
-        case 192: return line_fft_len<TC, 192, INV>(x, y, out, tw, lines, s);
-        case 384: return line_fft_len<TC, 384, INV>(x, y, out, tw, lines, s);
-        case 768: return line_fft_len<TC, 768, INV>(x, y, out, tw, lines, s);
-        case 1536: return line_fft_len<TC, 1536, INV>(x, y, out, tw, lines, s);
-        case 6144: return line_fft_len<TC, 6144, INV>(x, y, out, tw, lines, s);
-        case 12288: return line_fft_len<TC, 12288, INV>(x, y, out, tw, lines, s);
-        case 24576: return line_fft_len<TC, 24576, INV>(x, y, out, tw, lines, s);
-        case 44100: return line_fft_len<TC, 44100, INV>(x, y, out, tw, lines, s);
-        default: return hipErrorInvalidValue;
+    constexpr int L = L1 * L2, LS = padded_len(L2);
+    __shared__ cd lds[C * LS];
+    const size_t line = blockIdx.y;
+    const int k10 = blockIdx.x * C;
+    const cd *src = y + line * L;
+    for (int e = threadIdx.x; e < C * L2; e += kT) {
+        const int c = e / L2, n2 = e % L2;
+        lds[c * LS + pidx(n2)] = src[(size_t)(k10 + c) * L2 + n2];
+    }
+    __syncthreads();
+    fft_lines<L2, LS, C, kT, INV>(lds, tw, L1);
+    cd *dst = out + line * L;
+    for (int e = threadIdx.x; e < C * L2; e += kT) {
+        const int c = e % C, k2 = e / C;
+        dst[(size_t)k2 * L1 + k10 + c] = lds[c * LS + pidx(k2)];
+    }
+}
+
+__global__ void k_rows_phys(const cd *__restrict__ R, size_t n, double *__restrict__ out0, int out_mode)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const cd v = R[e];
+        out0[e] = out_mode == 1 ? sqrt(v.x * v.x + v.y * v.y) : v.x;
+    }
+}
+
+// E, Z partial sums per row iz of the [iz][kx] state (the energy convention of k_energy)
+__global__ __launch_bounds__(256) void k_energy_rows(const cd *__restrict__ om, int N, double *__restrict__ pe,
+                                                     double *__restrict__ pz)
+{
+    __shared__ double re[4], rz[4];
+    const int H = N / 2, iz = blockIdx.x;
+    const double gz = gam(iz, N);
+    double e = 0.0, zs = 0.0;
+    for (int kx = threadIdx.x; kx < H; kx += 256) {
+        const cd o = om[(size_t)iz * H + kx];
+        const double a2 = o.x * o.x + o.y * o.y, m = kx == 0 ? 1.0 : 2.0;
+        const double k2 = double(kx) * kx + gz * gz;
+        zs += m * a2;
+        if (k2 > 0.0) e += m * a2 / k2;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off, 64);
+        zs += __shfl_xor(zs, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        re[threadIdx.x >> 6] = e;
+        rz[threadIdx.x >> 6] = zs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        pe[iz] = re[0] + re[1] + re[2] + re[3];
+        pz[iz] = rz[0] + rz[1] + rz[2] + rz[3];
     }
 }
 
 int grid_for(size_t n) { return (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536); }
 
-template <class TC>
-hipError_t launch_big_t(const BigReq &r)
+template <class TC, int M_>
+hipError_t launch_m(const BigReq &r)
 {
-    const int H = r.H, M = 3 * H;
+    using S = Split<M_>;
+    constexpr int BZ = bz_of<TC>();
+    const int H = r.H;
+    const unsigned tiles = (unsigned)((H + BZ - 1) / BZ);
     const TC *tw = static_cast<const TC *>(r.tw);
     hipStream_t s = r.stream;
     switch (r.what) {
-        case BIG_SPEC:
-            hipLaunchKernelGGL(k_big_spec<TC>, dim3(grid_for((size_t)H * M)), dim3(256), 0, s, r.om,
-                               static_cast<TC *>(r.a), r.spec, H);
-            return hipGetLastError();
-        case BIG_FFT_INV:
-            return line_fft<TC, true>(M, static_cast<const TC *>(r.a), static_cast<TC *>(r.scratch),
-                                      static_cast<TC *>(r.b), tw, r.lines, s);
-        case BIG_FFT_FWD:
-            return line_fft<TC, false>(M, static_cast<const TC *>(r.a), static_cast<TC *>(r.scratch),
-                                       static_cast<TC *>(r.b), tw, r.lines, s);
-        case BIG_PACK: {
-            hipLaunchKernelGGL(k_big_zero_band<TC>, dim3(grid_for((size_t)M * (H + 1))), dim3(256), 0, s,
-                               static_cast<TC *>(r.b), H, M);
-            hipError_t e = hipGetLastError();
-            if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(k_big_pack<TC>, dim3((H + 31) / 32, (M + 31) / 32), dim3(256), 0, s,
-                               static_cast<const TC *>(r.a), static_cast<const TC *>(r.a2), static_cast<TC *>(r.b),
-                               H);
-            return hipGetLastError();
+        case BIG_ZINV_A:
+            hipLaunchKernelGGL((k_zinv_a<TC, M_>), dim3(S::L2 * tiles * (unsigned)r.fields.pairs), dim3(kT), 0, s, r.om,
+                               static_cast<TC *>(r.b), tw, r.fields, H, r.hs);
+            break;
+        case BIG_ZINV_B:
+            hipLaunchKernelGGL((k_zinv_b<TC, M_>), dim3(S::L1 * tiles * (unsigned)r.fields.pairs), dim3(kT), 0, s,
+                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs);
+            break;
+        case BIG_XINV_A: {
+            constexpr int NPT = (S::L2 / 2 - 1 + BZ - 1) / BZ;
+            hipLaunchKernelGGL((k_xinv_a<TC, M_>), dim3((NPT + 1) * M_ * (unsigned)r.planes), dim3(kT), 0, s,
+                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs, r.ms);
+            break;
         }
-        case BIG_PROD:
-            hipLaunchKernelGGL(k_big_prod<TC>, dim3(M / 2), dim3(256), 0, s, static_cast<const TC *>(r.a),
-                               static_cast<const TC *>(r.a2), static_cast<TC *>(r.b), M, r.cflrow, r.DX, r.DZ);
-            return hipGetLastError();
-        case BIG_CFL:
-            hipLaunchKernelGGL(k_big_cfl<TC>, dim3(M), dim3(256), 0, s, static_cast<const TC *>(r.a), M, r.cflrow,
-                               r.DX, r.DZ);
-            return hipGetLastError();
-        case BIG_UNPACK:
-            hipLaunchKernelGGL(k_big_unpack<TC>, dim3((H + 31) / 32, (M / 2 + 31) / 32), dim3(256), 0, s,
-                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), H);
-            return hipGetLastError();
-        case BIG_CN:
-            hipLaunchKernelGGL(k_big_cn<TC>, dim3(grid_for((size_t)H * 2 * H)), dim3(256), 0, s, r.om, r.fn,
-                               static_cast<const TC *>(r.a), r.cn, H);
-            return hipGetLastError();
-        case BIG_PHYS:
-            hipLaunchKernelGGL(k_big_phys<TC>, dim3(grid_for((size_t)M * M)), dim3(256), 0, s,
-                               static_cast<const TC *>(r.a), (size_t)M * M, r.out0, r.out1, r.out_mode);
-            return hipGetLastError();
+        case BIG_XINV_B:
+            hipLaunchKernelGGL((k_xinv_b<TC, M_, 0>), dim3(M_ / 2, S::L1 / S::C4), dim3(kT), 0, s,
+                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, r.ms, nullptr, 0.0, 0.0);
+            break;
+        case BIG_XINV_CFL:
+            hipLaunchKernelGGL((k_xinv_b<TC, M_, 1>), dim3(M_ / 2, S::L1 / S::C4), dim3(kT), 0, s,
+                               static_cast<const TC *>(r.a), nullptr, tw, r.ms, r.cflrow, r.DX, r.DZ);
+            break;
+        case BIG_XFWD_B:
+            hipLaunchKernelGGL((k_xfwd_b<TC, M_>), dim3((M_ / 2) * (S::L2 / S::C5)), dim3(kT), 0, s,
+                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.ms);
+            break;
+        case BIG_ZFWD_A:
+            hipLaunchKernelGGL((k_zfwd_a<TC, M_>), dim3((S::L2 / 2) * tiles), dim3(kT), 0, s,
+                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs, r.ms);
+            break;
+        case BIG_ZFWD_B:
+            hipLaunchKernelGGL((k_zfwd_b<TC, M_>), dim3(S::L1 * tiles), dim3(kT), 0, s, static_cast<const TC *>(r.a),
+                               r.om, r.fn, tw, H, r.hs, r.cn);
+            break;
         default:
             return hipErrorInvalidValue;
     }
+    return hipGetLastError();
 }
+
+// natural-order inverse line FFTs for the diagnostics (FP64)
+template <int M_>
+hipError_t launch_lines_m(const BigReq &r)
+{
+    using S = Split<M_>;
+    constexpr int L1 = S::L1, L2 = S::L2;
+    constexpr int C = (L1 % 8 == 0 && L2 % 8 == 0) ? 8 : ((L1 % 6 == 0 && L2 % 6 == 0) ? 6 : 2);
+    const cd *tw = static_cast<const cd *>(r.tw);
+    hipLaunchKernelGGL((k_line_a<L1, L2, C, true>), dim3(L2 / C, (unsigned)r.lines), dim3(kT), 0, r.stream,
+                       static_cast<const cd *>(r.a), static_cast<cd *>(r.scratch), tw);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_line_b<L1, L2, C, true>), dim3(L1 / C, (unsigned)r.lines), dim3(kT), 0, r.stream,
+                       static_cast<const cd *>(r.scratch), static_cast<cd *>(r.b), tw);
+    return hipGetLastError();
+}
+
+template <class TC>
+hipError_t launch_t(const BigReq &r, int M)
+{
+    switch (M) {
+        case 96: return launch_m<TC, 96>(r);
+        case 192: return launch_m<TC, 192>(r);
+        case 384: return launch_m<TC, 384>(r);
+        case 768: return launch_m<TC, 768>(r);
+        case 1536: return launch_m<TC, 1536>(r);
+        case 6144: return launch_m<TC, 6144>(r);
+        case 12288: return launch_m<TC, 12288>(r);
+        case 24576: return launch_m<TC, 24576>(r);
+        case 44100: return launch_m<TC, 44100>(r);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_lines(const BigReq &r, int M)
+{
+    switch (M) {
+        case 96: return launch_lines_m<96>(r);
+        case 192: return launch_lines_m<192>(r);
+        case 384: return launch_lines_m<384>(r);
+        case 768: return launch_lines_m<768>(r);
+        case 1536: return launch_lines_m<1536>(r);
+        case 6144: return launch_lines_m<6144>(r);
+        case 12288: return launch_lines_m<12288>(r);
+        case 24576: return launch_lines_m<24576>(r);
+        case 44100: return launch_lines_m<44100>(r);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int M_>
+constexpr int cfl_tiles_of() { return Split<M_>::L1 / Split<M_>::C4; }
 
 }  // namespace
 
@@ -417,11 +721,44 @@ int dns_big_supported_m(int M)
     switch (M) {
         case 96: case 192: case 384: case 768: case 1536: case 6144: case 12288: case 24576: case 44100:
             return 1;
+        default:
+            return 0;
+    }
+}
+
+int dns_big_cfl_tiles(int M)
+{
+    switch (M) {
+        case 96: return cfl_tiles_of<96>();
+        case 192: return cfl_tiles_of<192>();
+        case 384: return cfl_tiles_of<384>();
+        case 768: return cfl_tiles_of<768>();
+        case 1536: return cfl_tiles_of<1536>();
+        case 6144: return cfl_tiles_of<6144>();
+        case 12288: return cfl_tiles_of<12288>();
+        case 24576: return cfl_tiles_of<24576>();
+        case 44100: return cfl_tiles_of<44100>();
         default: return 0;
     }
 }
 
 hipError_t dns_big_launch(const BigReq &r)
 {
-    return r.fp32 ? launch_big_t<cf>(r) : launch_big_t<cd>(r);
+    const int M = 3 * r.H;
+    switch (r.what) {
+        case BIG_LINE_INV: return launch_lines(r, M);
+        case BIG_ROWS:
+            hipLaunchKernelGGL(k_rows, dim3(grid_for((size_t)M * M)), dim3(256), 0, r.stream,
+                               static_cast<const cd *>(r.a), static_cast<cd *>(r.b), r.H, M, r.hs, r.planes);
+            return hipGetLastError();
+        case BIG_ROWS_PHYS:
+            hipLaunchKernelGGL(k_rows_phys, dim3(grid_for((size_t)M * M)), dim3(256), 0, r.stream,
+                               static_cast<const cd *>(r.a), (size_t)M * M, r.out0, r.out_mode);
+            return hipGetLastError();
+        case BIG_ENERGY:
+            hipLaunchKernelGGL(k_energy_rows, dim3(2 * r.H), dim3(256), 0, r.stream, r.om, 2 * r.H, r.out0, r.out1);
+            return hipGetLastError();
+        default:
+            return r.fp32 ? launch_t<cf>(r, M) : launch_t<cd>(r, M);
+    }
 }
