@@ -157,9 +157,10 @@ def ensemble_leg(steps: int = 2000) -> dict:
     import optimizec_amd as oa
 
     members = ensemble_members()
-    e = oa.Ensemble(members, pair_every=20, filter_ratio=8, max_pairs=steps // 20)
+    e = oa.Ensemble(members, pair_every=20, filter_ratio=8, max_pairs=steps // 20, autotune=True)
     e.init()
     e.synchronize()
+    tuning = {N: e.tuning(next(i for i, x in enumerate(members) if x[0] == N)) for N in (256, 512)}
     t0 = time.perf_counter()
     e.advance(steps)
     e.synchronize()
@@ -195,6 +196,8 @@ def ensemble_leg(steps: int = 2000) -> dict:
             "step_hbm_GBps": round(algo / wall / 1e9, 1),
             "pairs_per_member": count, "coarse_n": [n0, 64], "device_GB": round(dev / 1e9, 2),
             "one_by_one_estimate_s": round(seq, 2), "speedup_vs_one_by_one": round(seq / wall, 2),
+            "autotune": {f"N{N}": {"configuration": v, "ms_row_plus_column_inverse": [round(x, 4) for x in ms]}
+                         for N, (v, ms) in tuning.items()},
             "member0_final": {"it": s0.it, "t": s0.t, "dt": s0.dt}}
 
 

@@ -56,6 +56,8 @@ typedef struct {
     int                   max_pairs;     /* training pairs kept per member (device memory)        */
     int                   use_graph;     /* 1: capture a block of pair_every steps in a HIP graph */
     int                   device;
+    int                   autotune;      /* 1: dns_ens_init times the built launch configurations
+                                            of each size and keeps the fastest               */
 } dns_ens_config;
 
 /* Allocates every member's state and work arrays on the device; no IC yet. */
@@ -81,6 +83,10 @@ dns_status dns_ens_get_fnm1(dns_ens *e, int m, double *host);
 dns_status dns_ens_pair_info(dns_ens *e, int m, int *count, int *n);
 /* Pairs first .. first + count - 1 of member m into host: count x 4 x n x n floats. */
 dns_status dns_ens_get_pairs(dns_ens *e, int m, int first, int count, float *host);
+/* Launch configuration chosen for member m's size (0 = default) and the time of one
+ * batched row + column-inverse pass of each built configuration (ms[i], i < *n; zeros
+ * without autotune).  Every configuration gives bitwise identical results. */
+dns_status dns_ens_tuning(dns_ens *e, int m, int *variant, double *ms, int *n);
 /* Device bytes held by the ensemble. */
 size_t dns_ens_device_bytes(const dns_ens *e);
 

@@ -60,7 +60,7 @@ class EnsMember(ctypes.Structure):
 class EnsConfig(ctypes.Structure):
     _fields_ = [("nmembers", ctypes.c_int), ("members", ctypes.POINTER(EnsMember)),
                 ("pair_every", ctypes.c_int), ("filter_ratio", ctypes.c_int), ("max_pairs", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("device", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("device", ctypes.c_int), ("autotune", ctypes.c_int)]
 
 
 _lib = None
@@ -117,6 +117,8 @@ SIGNATURES = {
     "dns_ens_get_pairs": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_float)]),
     "dns_ens_device_bytes": (ctypes.c_size_t, [_P]),
+    "dns_ens_tuning": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _D,
+                                      ctypes.POINTER(ctypes.c_int)]),
     # closure network (include/dns_closure.h)
     "dns_cnn_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_P)]),
     "dns_cnn_destroy": (None, [_P]),
@@ -348,7 +350,7 @@ class Ensemble:
     N / ``filter_ratio`` grid, channels (filtered omega, u, w, subgrid term)."""
 
     def __init__(self, members, pair_every: int = 20, filter_ratio: int = 8, max_pairs: int = 100,
-                 use_graph: bool = True, device: int = 0):
+                 use_graph: bool = True, device: int = 0, autotune: bool = False):
         L = load_library()
         self.members = [tuple(m) for m in members]
         arr = (EnsMember * len(self.members))()
@@ -356,7 +358,7 @@ class Ensemble:
             arr[i] = EnsMember(int(N), float(Re), float(K0), float(cfl), int(seed))
         self._arr = arr
         cfg = EnsConfig(len(self.members), arr, int(pair_every), int(filter_ratio), int(max_pairs),
-                        1 if use_graph else 0, int(device))
+                        1 if use_graph else 0, int(device), 1 if autotune else 0)
         h = _P()
         st = L.dns_ens_create(ctypes.byref(cfg), ctypes.byref(h))
         if st:
@@ -427,6 +429,13 @@ class Ensemble:
 
     def device_bytes(self) -> int:
         return int(load_library().dns_ens_device_bytes(self._h))
+
+    def tuning(self, m: int):
+        """(chosen launch configuration, [ms of each configuration]) for member m's size."""
+        v, n = ctypes.c_int(), ctypes.c_int(8)
+        ms = (ctypes.c_double * 8)()
+        self._check(load_library().dns_ens_tuning(self._h, m, ctypes.byref(v), ms, ctypes.byref(n)))
+        return v.value, [ms[i] for i in range(min(8, n.value))]
 
 
 def _fptr(a: np.ndarray):

@@ -37,6 +37,8 @@ struct Group {
     EnsScalars *sc = nullptr;
     cd *om_c = nullptr, *fn_c = nullptr, *V_c = nullptr, *G_c = nullptr, *tw_c = nullptr;
     float *pairs = nullptr;
+    int variant = 0;          // launch configuration of the row / column kernels
+    double tune_ms[4] = {0, 0, 0, 0};
 };
 
 }  // namespace
@@ -44,7 +46,7 @@ struct Group {
 struct dns_ens {
     int device = 0;
     hipStream_t stream = nullptr;
-    int pair_every = 20, ratio = 8, max_pairs = 0, use_graph = 1;
+    int pair_every = 20, ratio = 8, max_pairs = 0, use_graph = 1, autotune = 0;
     std::vector<dns_ens_member> members;
     std::vector<Group> groups;
     std::vector<std::pair<int, int>> where;  // member -> (group, slot)
@@ -190,8 +192,12 @@ dns_status klaunch(dns_ens *e, int H, const LaunchReq &r)
 dns_status issue_step(dns_ens *e, Group &g)
 {
     EHIP(ens_launch_pre(g.sc, g.coef, g.count, e->stream));
-    ESTEP(klaunch(e, g.H, row_req(e, g.N, g.H, g.M, g.V, g.G, g.tw, nullptr, LAUNCH_ROW, batch_of(g, nullptr))));
-    ESTEP(klaunch(e, g.H, col_req(e, g.H, g.M, g.om, g.fn, g.G, g.V, g.tw, 1, batch_of(g, g.coef))));
+    LaunchReq rr = row_req(e, g.N, g.H, g.M, g.V, g.G, g.tw, nullptr, LAUNCH_ROW, batch_of(g, nullptr));
+    rr.variant = g.variant;
+    ESTEP(klaunch(e, g.H, rr));
+    LaunchReq cr = col_req(e, g.H, g.M, g.om, g.fn, g.G, g.V, g.tw, 1, batch_of(g, g.coef));
+    cr.variant = g.variant;
+    ESTEP(klaunch(e, g.H, cr));
     EHIP(ens_launch_post(g.sc, g.count, e->stream));
     return DNS_OK;
 }
@@ -276,6 +282,43 @@ dns_status refresh_group(dns_ens *e, Group &g)
     return klaunch(e, g.H, col_req(e, g.H, g.M, g.om, g.fn, nullptr, g.V, g.tw, 0, batch_of(g, nullptr)));
 }
 
+// Times one batched row pass + column inverse of each launch configuration of g's size
+// (state untouched: the row pass writes G only, the column inverse rewrites V with the
+// same values) and keeps the fastest.
+dns_status autotune_group(dns_ens *e, Group &g)
+{
+    const int nv = dns_kernels_variants(g.H, 0);
+    hipEvent_t a = nullptr, b = nullptr;
+    EHIP(hipEventCreate(&a));
+    EHIP(hipEventCreate(&b));
+    double best = 1e300;
+    for (int v = 0; v < nv; ++v) {
+        LaunchReq rr = row_req(e, g.N, g.H, g.M, g.V, g.G, g.tw, nullptr, LAUNCH_ROW, batch_of(g, nullptr));
+        LaunchReq cr = col_req(e, g.H, g.M, g.om, g.fn, nullptr, g.V, g.tw, 0, batch_of(g, nullptr));
+        rr.variant = cr.variant = v;
+        ESTEP(klaunch(e, g.H, rr));  // warm-up
+        ESTEP(klaunch(e, g.H, cr));
+        EHIP(hipEventRecord(a, e->stream));
+        constexpr int reps = 3;
+        for (int k = 0; k < reps; ++k) {
+            ESTEP(klaunch(e, g.H, rr));
+            ESTEP(klaunch(e, g.H, cr));
+        }
+        EHIP(hipEventRecord(b, e->stream));
+        EHIP(hipEventSynchronize(b));
+        float ms = 0.0f;
+        EHIP(hipEventElapsedTime(&ms, a, b));
+        g.tune_ms[v] = ms / reps;
+        if (g.tune_ms[v] < best) {
+            best = g.tune_ms[v];
+            g.variant = v;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return DNS_OK;
+}
+
 }  // namespace
 
 extern "C" const char *dns_ens_last_error(const dns_ens *e) { return e ? e->err.c_str() : g_ens_create_error.c_str(); }
@@ -322,6 +365,7 @@ extern "C" dns_status dns_ens_create(const dns_ens_config *cfg, dns_ens **out)
     e->ratio = cfg->filter_ratio;
     e->max_pairs = cfg->max_pairs;
     e->use_graph = cfg->use_graph ? 1 : 0;
+    e->autotune = cfg->autotune ? 1 : 0;
     e->members.assign(cfg->members, cfg->members + cfg->nmembers);
     auto bail = [&](dns_status st) {
         g_ens_create_error = e->err;
@@ -433,6 +477,7 @@ extern "C" dns_status dns_ens_init(dns_ens *e)
         }
         EHIP(hipMemcpy(g.sc, sc.data(), sizeof(EnsScalars) * g.count, hipMemcpyHostToDevice));
         ESTEP(refresh_group(e, g));
+        if (e->autotune) ESTEP(autotune_group(e, g));
         ESTEP(issue_nextdt(e, g));
     }
     EHIP(hipMemsetAsync(e->slot_d, 0, sizeof(int), e->stream));
@@ -575,6 +620,20 @@ extern "C" dns_status dns_ens_get_pairs(dns_ens *e, int m, int first, int count,
         EHIP(hipMemcpyAsync(host + (size_t)p * per, g->pairs + ((size_t)(first + p) * g->count + slot) * per,
                             sizeof(float) * per, hipMemcpyDeviceToHost, e->stream));
     EHIP(hipStreamSynchronize(e->stream));
+    return DNS_OK;
+}
+
+extern "C" dns_status dns_ens_tuning(dns_ens *e, int m, int *variant, double *ms, int *n)
+{
+    if (!e) return DNS_E_ARG;
+    Group *g = nullptr;
+    int slot = 0;
+    ESTEP(locate(e, m, &g, &slot));
+    const int nv = dns_kernels_variants(g->H, 0);
+    if (variant) *variant = g->variant;
+    if (ms && n)
+        for (int i = 0; i < std::min(*n, nv); ++i) ms[i] = g->tune_ms[i];
+    if (n) *n = nv;
     return DNS_OK;
 }
 

@@ -61,9 +61,13 @@ struct LaunchReq {
     double *out0, *out1;
     int out_mode;
     BatchArgs batch;           // column and row kernels (MODE 0, 1)
+    int variant;               // launch configuration (0 default; see dns_kernels_variants)
 };
 
 int dns_kernels_supported(int H);
+// number of launch configurations of the row / column kernels built for H (variant
+// 0 .. n-1 of LaunchReq); all give bitwise identical results
+int dns_kernels_variants(int H, int fp32);
 int dns_kernels_col_tile(int H);
 hipError_t dns_kernels_launch(int H, const LaunchReq &r);
 hipError_t dns_launch_energy(const dnsfft::cd *om, int N, int H, double *partE, double *partZ, hipStream_t s);

@@ -27,6 +27,8 @@
 //                                    step2a_cuda.cu:59-261 + cuFFT C2R]
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cstddef>
 #include <cstdint>
 
@@ -591,10 +593,41 @@ constexpr int min_waves()
     return 1;
 }
 
+// Launch variants (the ensemble's autotuner times them): the column tile and the
+// column-kernel threads scaled by CM, the row-kernel threads by TM.  Every variant does
+// the same arithmetic per element (only which thread does it changes), so results are
+// bitwise those of variant 0.
+template <int H, int CM, int TM>
+struct VarCfg {
+    static constexpr int C = Cfg<H>::C * CM;
+    static constexpr int TC = Cfg<H>::TC * CM;
+    static constexpr int TR = Cfg<H>::TR * TM;
+};
+// variants are built for FP64 and H a power of two in 16 .. 512 (the ensemble sizes)
+constexpr bool has_variants(int H) { return H >= 16 && H <= 512 && (H & (H - 1)) == 0; }
+
+template <class TC, int H, class Q>
+hipError_t launch_cfg(const LaunchReq &r);
+
 template <class TC, int H>
 hipError_t launch_prec(const LaunchReq &r)
 {
-    using Q = Cfg<H>;
+    if constexpr (has_variants(H) && std::is_same<TC, cd>::value) {
+        {
+            switch (r.variant) {
+                case 1: return launch_cfg<TC, H, VarCfg<H, 2, 1>>(r);
+                case 2: return launch_cfg<TC, H, VarCfg<H, 1, 2>>(r);
+                case 3: return launch_cfg<TC, H, VarCfg<H, 2, 2>>(r);
+                default: break;
+            }
+        }
+    }
+    return launch_cfg<TC, H, Cfg<H>>(r);
+}
+
+template <class TC, int H, class Q>
+hipError_t launch_cfg(const LaunchReq &r)
+{
     constexpr int M = 3 * H;
     constexpr int WC = min_waves<TC, H, Q::TC, 3 * Q::C * padded_len(H) * (int)sizeof(TC)>();
     constexpr int WR = min_waves<TC, H, Q::TR, 3 * padded_len(H) * (int)sizeof(TC)>();
@@ -636,6 +669,11 @@ hipError_t launch_all(const LaunchReq &r)
 }
 
 }  // namespace
+
+int dns_kernels_variants(int H, int fp32)
+{
+    return (!fp32 && has_variants(H) && dns_kernels_supported(H)) ? 4 : 1;
+}
 
 int dns_kernels_supported(int H)
 {

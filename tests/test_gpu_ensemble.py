@@ -34,11 +34,20 @@ def _standalone(m, steps, record=()):
     return s, states
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_members_bitwise_equal_single_runs(use_graph):
-    print(f"ensemble vs single runs, graph={use_graph}", flush=True)
-    e = oa.Ensemble(MEMBERS, pair_every=EVERY, filter_ratio=RATIO, max_pairs=8, use_graph=use_graph)
+@pytest.mark.parametrize("use_graph,autotune", [(True, False), (False, False), (True, True)])
+def test_members_bitwise_equal_single_runs(use_graph, autotune):
+    """Also with the autotuner: every launch configuration does the same arithmetic per
+    element, so the chosen one changes nothing in the results."""
+    print(f"ensemble vs single runs, graph={use_graph}, autotune={autotune}", flush=True)
+    e = oa.Ensemble(MEMBERS, pair_every=EVERY, filter_ratio=RATIO, max_pairs=8, use_graph=use_graph,
+                    autotune=autotune)
     e.init()
+    if autotune:
+        for m in range(len(MEMBERS)):
+            v, ms = e.tuning(m)
+            print(f"member {m} (N={MEMBERS[m][0]}): configuration {v}, ms {ms}")
+            assert len(ms) == 4 and 0 <= v < 4 and all(x > 0 for x in ms)
+            assert ms[v] == min(ms)
     e.advance(40)
     e.synchronize()
     for m, mem in enumerate(MEMBERS):
