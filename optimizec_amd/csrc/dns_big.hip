@@ -39,7 +39,11 @@ using namespace dnsfft;
 
 namespace {
 
+// threads per workgroup: the diagnostics' line FFTs (kT) and the step's passes (kTZ: the
+// z-direction passes 1, 2, 6, 7; kTXA: pass 3; kTXB: pass 4; kTXF: pass 5), chosen by
+// timing at N = 29400
 constexpr int kT = 256;
+constexpr int kTZ = 512, kTXA = 512, kTXB = 256, kTXF = 512;
 
 __device__ __forceinline__ double gam(int iz, int N) { return (iz < (N >> 1)) ? double(iz) : double(iz - N); }
 
@@ -105,25 +109,61 @@ __device__ __forceinline__ TC twid(const TC *__restrict__ tw, int e)
     return INV ? conj(w) : w;
 }
 
+// Twiddles in LDS.  The passes take every twiddle from small LDS tables filled at kernel
+// start from the W_M table, so no global load inside a transform waits behind the pass's
+// own data loads (and the inter-step factors no longer miss in L1):
+//   W_L^j of an L-point line FFT (L1: tw[j L2]; L2: tw[j L1]), L entries;
+//   the inter-step factor W_M^e, e = n k < M, as W_M^{(e / L1) L1} W_M^{e % L1}: the
+//   tables W_L2^a = tw[a L1] (a < L2, the L2 line table) and W_M^b = tw[b] (b < L1).
+// A thread fetches its table entry into a register before issuing its data loads and
+// writes it to LDS after them, so the table and data loads are in flight together.
+template <class TC, int N, int T>
+struct TabFetch {
+    static_assert(N <= T, "one table entry per thread");
+    TC v;
+    __device__ __forceinline__ void fetch(const TC *__restrict__ tw, int stride)
+    {
+        v = threadIdx.x < N ? tw[threadIdx.x * stride] : TC{0, 0};
+    }
+    __device__ __forceinline__ void put(TC *dst) const
+    {
+        if (threadIdx.x < N) dst[threadIdx.x] = v;
+    }
+};
+
+template <class TC, int L1, bool INV>
+__device__ __forceinline__ TC twid2(const TC *wa, const TC *wb, int e)
+{
+    const TC w = mul(wa[e / L1], wb[e % L1]);
+    return INV ? conj(w) : w;
+}
+
 // ---- pass 1: om -> two field spectra on the padded z points, z pass A ------------------
-template <class TC, int M_>
-__global__ __launch_bounds__(kT) void k_zinv_a(const cd *__restrict__ om, TC *__restrict__ Y, const TC *__restrict__ tw,
+template <class TC, int M_, int T>
+__global__ __launch_bounds__(T) void k_zinv_a(const cd *__restrict__ om, TC *__restrict__ Y, const TC *__restrict__ tw,
                                                BigFields fl, int H, size_t hs)
 {
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, BZ = bz_of<TC>(), LS = padded_len(L1);
     __shared__ TC lds[2 * BZ * LS];
+    __shared__ TC w1[L1], w2[L2], wb[L1];
     const int N = 2 * H;
     const int tiles = (H + BZ - 1) / BZ;
     const int lb = xcd_order(blockIdx.x, gridDim.x);
     const int kx0 = (lb % tiles) * BZ, n2 = (lb / tiles) % L2, pr = lb / (tiles * L2);
     const int fa = fl.f[2 * pr], fb = fl.f[2 * pr + 1];
     // all of a thread's loads first (independent, in flight together), then the LDS writes
-    constexpr int CNT = BZ * L1, IT = (CNT + kT - 1) / kT;
+    constexpr int CNT = BZ * L1, IT = (CNT + T - 1) / T;
+    TabFetch<TC, L1, T> tf1;
+    TabFetch<TC, L2, T> tf2;
+    TabFetch<TC, L1, T> tfb;
+    tf1.fetch(tw, L2);
+    tf2.fetch(tw, L1);
+    tfb.fetch(tw, 1);
     cd ov[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         const int c = e % BZ, n1 = e / BZ;
         const int kx = kx0 + c, zp = n1 * L2 + n2;
         const int iz = zp < H ? zp : (zp >= 2 * H ? zp - H : -1);
@@ -131,7 +171,7 @@ __global__ __launch_bounds__(kT) void k_zinv_a(const cd *__restrict__ om, TC *__
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         if (e < CNT) {
             const int c = e % BZ, n1 = e / BZ;
             const int kx = kx0 + c, zp = n1 * L2 + n2;
@@ -145,14 +185,17 @@ __global__ __launch_bounds__(kT) void k_zinv_a(const cd *__restrict__ om, TC *__
             lds[(BZ + c) * LS + pidx(n1)] = vb;
         }
     }
+    tf1.put(w1);
+    tf2.put(w2);
+    tfb.put(wb);
     __syncthreads();
-    fft_lines<L1, LS, 2 * BZ, kT, true>(lds, tw, L2);
+    fft_lines<L1, LS, 2 * BZ, T, true>(lds, w1, 1);
     TC *ya = Y + (size_t)(2 * pr) * M_ * hs, *yb = ya + (size_t)M_ * hs;
-    for (int e = threadIdx.x; e < BZ * L1; e += kT) {
+    for (int e = threadIdx.x; e < BZ * L1; e += T) {
         const int c = e % BZ, k1 = e / BZ;
         const int kx = kx0 + c;
         if (kx < H) {
-            const TC w = twid<TC, true>(tw, n2 * k1);
+            const TC w = twid2<TC, L1, true>(w2, wb, n2 * k1);
             const size_t o = (size_t)(k1 * L2 + n2) * hs + kx;
             ya[o] = mul(lds[c * LS + pidx(k1)], w);
             yb[o] = mul(lds[(BZ + c) * LS + pidx(k1)], w);
@@ -161,22 +204,25 @@ __global__ __launch_bounds__(kT) void k_zinv_a(const cd *__restrict__ om, TC *__
 }
 
 // ---- pass 2: z pass B -> fields on the physical rows ------------------------------------
-template <class TC, int M_>
-__global__ __launch_bounds__(kT) void k_zinv_b(const TC *__restrict__ Y, TC *__restrict__ Z, const TC *__restrict__ tw,
+template <class TC, int M_, int T>
+__global__ __launch_bounds__(T) void k_zinv_b(const TC *__restrict__ Y, TC *__restrict__ Z, const TC *__restrict__ tw,
                                                int H, size_t hs)
 {
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, BZ = bz_of<TC>(), LS = padded_len(L2);
     __shared__ TC lds[2 * BZ * LS];
+    __shared__ TC w2[L2];
     const int tiles = (H + BZ - 1) / BZ;
     const int lb = xcd_order(blockIdx.x, gridDim.x);
     const int kx0 = (lb % tiles) * BZ, k1 = (lb / tiles) % L1, pr = lb / (tiles * L1);
     const TC *ya = Y + (size_t)(2 * pr) * M_ * hs, *yb = ya + (size_t)M_ * hs;
-    constexpr int CNT = BZ * L2, IT = (CNT + kT - 1) / kT;
+    constexpr int CNT = BZ * L2, IT = (CNT + T - 1) / T;
+    TabFetch<TC, L2, T> tf2;
+    tf2.fetch(tw, L1);
     TC va[IT], vb[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         const int c = e % BZ, n2 = e / BZ;
         const int kx = kx0 + c;
         const size_t o = (size_t)(k1 * L2 + n2) * hs + kx;
@@ -186,17 +232,18 @@ __global__ __launch_bounds__(kT) void k_zinv_b(const TC *__restrict__ Y, TC *__r
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         if (e < CNT) {
             const int c = e % BZ, n2 = e / BZ;
             lds[c * LS + pidx(n2)] = va[i];
             lds[(BZ + c) * LS + pidx(n2)] = vb[i];
         }
     }
+    tf2.put(w2);
     __syncthreads();
-    fft_lines<L2, LS, 2 * BZ, kT, true>(lds, tw, L1);
+    fft_lines<L2, LS, 2 * BZ, T, true>(lds, w2, 1);
     TC *za = Z + (size_t)(2 * pr) * M_ * hs, *zb = za + (size_t)M_ * hs;
-    for (int e = threadIdx.x; e < BZ * L2; e += kT) {
+    for (int e = threadIdx.x; e < BZ * L2; e += T) {
         const int c = e % BZ, k2 = e / BZ;
         const int kx = kx0 + c;
         if (kx < H) {
@@ -231,26 +278,33 @@ __device__ __forceinline__ TC row_spec(const TC *__restrict__ a, const TC *__res
 // columns 0 and L2/2.  Positions kx and M - kx then both lie in its lines, so each
 // retained value of the row is read once and fills its direct slot (a + i b) and its
 // mirror slot (conj a + i conj b); the band H <= kx <= M - H is zero.
-template <class TC, int M_>
-__global__ __launch_bounds__(kT) void k_xinv_a(const TC *__restrict__ Z, TC *__restrict__ X, const TC *__restrict__ tw,
+template <class TC, int M_, int T>
+__global__ __launch_bounds__(T) void k_xinv_a(const TC *__restrict__ Z, TC *__restrict__ X, const TC *__restrict__ tw,
                                                int H, size_t hs, size_t ms)
 {
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, CP = bz_of<TC>(), LS = padded_len(L1);
     constexpr int NPT = (L2 / 2 - 1 + CP - 1) / CP;  // pair tiles; tile NPT holds columns 0, L2/2
     __shared__ TC lds[2 * CP * LS];
+    __shared__ TC w1[L1], w2[L2], wb[L1];
     const int lb = xcd_order(blockIdx.x, gridDim.x);
     const int t = lb % (NPT + 1), z = (lb / (NPT + 1)) % M_, p = lb / ((NPT + 1) * M_);
     const TC *a = Z + ((size_t)(2 * p) * M_ + z) * hs;
     const TC *b = a + (size_t)M_ * hs;
-    constexpr int CNT = CP * L1, IT = (CNT + kT - 1) / kT;
+    constexpr int CNT = CP * L1, IT = (CNT + T - 1) / T;
+    TabFetch<TC, L1, T> tf1;
+    TabFetch<TC, L2, T> tf2;
+    TabFetch<TC, L1, T> tfb;
+    tf1.fetch(tw, L2);
+    tf2.fetch(tw, L1);
+    tfb.fetch(tw, 1);
     TC va[IT], vb[IT];
     if (t < NPT) {
         // slot pair (line i, n1) = position kp = n1 L2 + d, d = 1 + t CP + i, and (line CP + i,
         // L1 - 1 - n1) = position M - kp; the smaller of the two is the retained one
 #pragma unroll
         for (int k = 0; k < IT; ++k) {
-            const int e = threadIdx.x + k * kT;
+            const int e = threadIdx.x + k * T;
             const int i = e % CP, n1 = e / CP;
             const int d = 1 + t * CP + i;
             const int kp = n1 * L2 + d, lo = kp < M_ - kp ? kp : M_ - kp;
@@ -260,23 +314,24 @@ __global__ __launch_bounds__(kT) void k_xinv_a(const TC *__restrict__ Z, TC *__r
         }
 #pragma unroll
         for (int k = 0; k < IT; ++k) {
-            const int e = threadIdx.x + k * kT;
+            const int e = threadIdx.x + k * T;
             if (e < CNT) {
                 const int i = e % CP, n1 = e / CP;
                 const int kp = n1 * L2 + 1 + t * CP + i;
                 const TC av = va[k], bv = vb[k];
                 const TC dir{av.x - bv.y, av.y + bv.x}, mir{av.x + bv.y, bv.x - av.y};
                 const bool first = kp < M_ - kp;  // this slot holds the retained position
-                lds[i * LS + pidx(n1)] = first ? dir : mir;
-                lds[(CP + i) * LS + pidx(L1 - 1 - n1)] = first ? mir : dir;
+                // component-wise selects (a select of the 16-byte structs goes through scratch)
+                lds[i * LS + pidx(n1)] = TC{first ? dir.x : mir.x, first ? dir.y : mir.y};
+                lds[(CP + i) * LS + pidx(L1 - 1 - n1)] = TC{first ? mir.x : dir.x, first ? mir.y : dir.y};
             }
         }
     } else {
         // columns 0 and L2/2 (lines 0, 1), each its own mirror; other lines unused
-        constexpr int CNT2 = 2 * L1, IT2 = (CNT2 + kT - 1) / kT;
+        constexpr int CNT2 = 2 * L1, IT2 = (CNT2 + T - 1) / T;
 #pragma unroll
         for (int k = 0; k < IT2; ++k) {
-            const int e = threadIdx.x + k * kT;
+            const int e = threadIdx.x + k * T;
             const int col = (e / L1) * (L2 / 2), n1 = e % L1;
             const int kp = n1 * L2 + col;
             const int src = kp < H ? kp : (kp > M_ - H ? M_ - kp : -1);
@@ -292,12 +347,15 @@ __global__ __launch_bounds__(kT) void k_xinv_a(const TC *__restrict__ Z, TC *__r
             }
             if (e < CNT2) lds[(e / L1) * LS + pidx(n1)] = v;
         }
-        for (int e = threadIdx.x; e < (2 * CP - 2) * L1; e += kT) lds[(2 + e / L1) * LS + pidx(e % L1)] = TC{0, 0};
+        for (int e = threadIdx.x; e < (2 * CP - 2) * L1; e += T) lds[(2 + e / L1) * LS + pidx(e % L1)] = TC{0, 0};
     }
+    tf1.put(w1);
+    tf2.put(w2);
+    tfb.put(wb);
     __syncthreads();
-    fft_lines<L1, LS, 2 * CP, kT, true>(lds, tw, L2);
+    fft_lines<L1, LS, 2 * CP, T, true>(lds, w1, 1);
     const int nl = t < NPT ? 2 * CP : 2;
-    for (int e = threadIdx.x; e < nl * L1; e += kT) {
+    for (int e = threadIdx.x; e < nl * L1; e += T) {
         const int l = e % nl, k1 = e / nl;
         int n2;
         if (t < NPT) {
@@ -307,45 +365,52 @@ __global__ __launch_bounds__(kT) void k_xinv_a(const TC *__restrict__ Z, TC *__r
         } else {
             n2 = l * (L2 / 2);
         }
-        X[((size_t)p * M_ + z) * ms + (size_t)k1 * L2 + n2] = mul(lds[l * LS + pidx(k1)], twid<TC, true>(tw, n2 * k1));
+        X[((size_t)p * M_ + z) * ms + (size_t)k1 * L2 + n2] = mul(lds[l * LS + pidx(k1)], twid2<TC, L1, true>(w2, wb, n2 * k1));
     }
 }
 
 // ---- pass 4: x pass B of a row pair, product, first forward step -------------------------
 // MODE 0: product -> Yq;  MODE 1: CFL maxima of u, w (plane 0 only) -> cflpart
-template <class TC, int M_, int MODE>
-__global__ __launch_bounds__(kT) void k_xinv_b(const TC *__restrict__ X, TC *__restrict__ Yq, const TC *__restrict__ tw,
+template <class TC, int M_, int MODE, int T>
+__global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__restrict__ Yq, const TC *__restrict__ tw,
                                                size_t ms, double *__restrict__ cflpart, double DX, double DZ)
 {
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, C = S::C4, LS = padded_len(L2);
     constexpr int G = MODE == 0 ? 4 : 2;  // lines per k1: group g = 2 p + r (plane p, row 2q + r)
     __shared__ TC lds[G * C * LS];
-    __shared__ double red[kT / 64];
+    __shared__ TC w2[L2], wb[L1];
+    __shared__ double red[T / 64];
     using RT = decltype(TC::x);
     const int q = blockIdx.x, k10 = blockIdx.y * C;
-    constexpr int CNT = G * C * L2, IT = (CNT + kT - 1) / kT;
+    constexpr int CNT = G * C * L2, IT = (CNT + T - 1) / T;
+    TabFetch<TC, L2, T> tf2;
+    TabFetch<TC, L1, T> tfb;
+    tf2.fetch(tw, L1);
+    tfb.fetch(tw, 1);
     TC xv[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         const int n2 = e % L2, rest = e / L2, c = rest % C, g = rest / C;
         const int p = g >> 1, r = g & 1;
         xv[i] = e < CNT ? X[((size_t)p * M_ + 2 * q + r) * ms + (size_t)(k10 + c) * L2 + n2] : TC{0, 0};
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         if (e < CNT) {
             const int n2 = e % L2, rest = e / L2, c = rest % C, g = rest / C;
             lds[(g * C + c) * LS + pidx(n2)] = xv[i];
         }
     }
+    tf2.put(w2);
+    tfb.put(wb);
     __syncthreads();
-    fft_lines<L2, LS, G * C, kT, true>(lds, tw, L1);
+    fft_lines<L2, LS, G * C, T, true>(lds, w2, 1);
     if (MODE == 1) {
         double m = 0.0;
-        for (int e = threadIdx.x; e < 2 * C * L2; e += kT) {
+        for (int e = threadIdx.x; e < 2 * C * L2; e += T) {
             const TC v = lds[(e / L2) * LS + pidx(e % L2)];
             const double cfl = fabs(double(v.x)) / DX + fabs(double(v.y)) / DZ;
             m = cfl > m ? cfl : m;
@@ -358,31 +423,31 @@ __global__ __launch_bounds__(kT) void k_xinv_b(const TC *__restrict__ X, TC *__r
         __syncthreads();
         if (threadIdx.x == 0) {
             double t = red[0];
-            for (int w = 1; w < kT / 64; ++w) t = red[w] > t ? red[w] : t;
+            for (int w = 1; w < T / 64; ++w) t = red[w] > t ? red[w] : t;
             cflpart[(size_t)q * gridDim.y + blockIdx.y] = t;
         }
         return;
     }
     // P = u w_x + w w_z on rows 2q (real part) and 2q + 1 (imaginary part): group 0/1 hold
     // u + i w of the two rows, group 2/3 w_x + i w_z
-    for (int e = threadIdx.x; e < C * L2; e += kT) {
+    for (int e = threadIdx.x; e < C * L2; e += T) {
         const int c = e / L2, k2 = e % L2;
         const TC a0 = lds[(0 * C + c) * LS + pidx(k2)], a1 = lds[(1 * C + c) * LS + pidx(k2)];
         const TC b0 = lds[(2 * C + c) * LS + pidx(k2)], b1 = lds[(3 * C + c) * LS + pidx(k2)];
         lds[c * LS + pidx(k2)] = TC{RT(a0.x * b0.x + a0.y * b0.y), RT(a1.x * b1.x + a1.y * b1.y)};
     }
     __syncthreads();
-    fft_lines<L2, LS, C, kT, false>(lds, tw, L1);
-    for (int e = threadIdx.x; e < C * L2; e += kT) {
+    fft_lines<L2, LS, C, T, false>(lds, w2, 1);
+    for (int e = threadIdx.x; e < C * L2; e += T) {
         const int c = e / L2, m2 = e % L2;
         const int k1 = k10 + c;
-        Yq[(size_t)q * ms + (size_t)k1 * L2 + m2] = mul(lds[c * LS + pidx(m2)], twid<TC, false>(tw, k1 * m2));
+        Yq[(size_t)q * ms + (size_t)k1 * L2 + m2] = mul(lds[c * LS + pidx(m2)], twid2<TC, L1, false>(w2, wb, k1 * m2));
     }
 }
 
 // ---- pass 5: L1-point FFTs over k1 -> packed row-pair spectra (kept modes only) ----------
-template <class TC, int M_>
-__global__ __launch_bounds__(kT) void k_xfwd_b(const TC *__restrict__ Yq, TC *__restrict__ Qh, const TC *__restrict__ tw,
+template <class TC, int M_, int T>
+__global__ __launch_bounds__(T) void k_xfwd_b(const TC *__restrict__ Yq, TC *__restrict__ Qh, const TC *__restrict__ tw,
                                                int H, size_t ms)
 {
     using S = Split<M_>;
@@ -390,22 +455,22 @@ __global__ __launch_bounds__(kT) void k_xfwd_b(const TC *__restrict__ Yq, TC *__
     __shared__ TC lds[C * LS];
     const int lb = xcd_order(blockIdx.x, gridDim.x);
     const int m20 = (lb % (L2 / C)) * C, q = lb / (L2 / C);
-    constexpr int CNT = C * L1, IT = (CNT + kT - 1) / kT;
+    constexpr int CNT = C * L1, IT = (CNT + T - 1) / T;
     TC yv[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         const int c = e % C, k1 = e / C;
         yv[i] = e < CNT ? Yq[(size_t)q * ms + (size_t)k1 * L2 + m20 + c] : TC{0, 0};
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         if (e < CNT) lds[(e % C) * LS + pidx(e / C)] = yv[i];
     }
     __syncthreads();
-    fft_lines<L1, LS, C, kT, false>(lds, tw, L2);
-    for (int e = threadIdx.x; e < C * L1; e += kT) {
+    fft_lines<L1, LS, C, T, false>(lds, tw, L2);  // global table: L1-cached stage twiddles
+    for (int e = threadIdx.x; e < C * L1; e += T) {
         const int c = e % C, m1 = e / C;
         const int kx = m20 + c + L2 * m1;
         if (kx < H || kx > M_ - H) Qh[(size_t)q * ms + kx] = lds[c * LS + pidx(m1)];
@@ -413,23 +478,30 @@ __global__ __launch_bounds__(kT) void k_xfwd_b(const TC *__restrict__ Yq, TC *__
 }
 
 // ---- pass 6: split the row pairs, z pass A (forward) -------------------------------------
-template <class TC, int M_>
-__global__ __launch_bounds__(kT) void k_zfwd_a(const TC *__restrict__ Qh, TC *__restrict__ W, const TC *__restrict__ tw,
+template <class TC, int M_, int T>
+__global__ __launch_bounds__(T) void k_zfwd_a(const TC *__restrict__ Qh, TC *__restrict__ W, const TC *__restrict__ tw,
                                                int H, size_t hs, size_t ms)
 {
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, BZ = bz_of<TC>(), LS = padded_len(L1);
     __shared__ TC lds[2 * BZ * LS];
+    __shared__ TC w1[L1], w2[L2], wb[L1];
     using RT = decltype(TC::x);
     const int tiles = (H + BZ - 1) / BZ;
     const int lb = xcd_order(blockIdx.x, gridDim.x);
     const int kx0 = (lb % tiles) * BZ, j = lb / tiles;  // n2 = 2 j + r: rows z of parity r
     const RT half = RT(0.5);
-    constexpr int CNT = BZ * L1, IT = (CNT + kT - 1) / kT;
+    constexpr int CNT = BZ * L1, IT = (CNT + T - 1) / T;
+    TabFetch<TC, L1, T> tf1;
+    TabFetch<TC, L2, T> tf2;
+    TabFetch<TC, L1, T> tfb;
+    tf1.fetch(tw, L2);
+    tf2.fetch(tw, L1);
+    tfb.fetch(tw, 1);
     TC zkv[IT], zmv[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         const int c = e % BZ, n1 = e / BZ;
         const int kx = kx0 + c;
         const size_t q = (size_t)(n1 * L2 + 2 * j) / 2;
@@ -439,7 +511,7 @@ __global__ __launch_bounds__(kT) void k_zfwd_a(const TC *__restrict__ Qh, TC *__
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         if (e < CNT) {
             const int c = e % BZ, n1 = e / BZ;
             const TC zk = zkv[i], zm = conj(zmv[i]);
@@ -448,45 +520,52 @@ __global__ __launch_bounds__(kT) void k_zfwd_a(const TC *__restrict__ Qh, TC *__
             lds[(BZ + c) * LS + pidx(n1)] = TC{half * df.y, -half * df.x};
         }
     }
+    tf1.put(w1);
+    tf2.put(w2);
+    tfb.put(wb);
     __syncthreads();
-    fft_lines<L1, LS, 2 * BZ, kT, false>(lds, tw, L2);
-    for (int e = threadIdx.x; e < 2 * BZ * L1; e += kT) {
+    fft_lines<L1, LS, 2 * BZ, T, false>(lds, w1, 1);
+    for (int e = threadIdx.x; e < 2 * BZ * L1; e += T) {
         const int c = e % BZ, rest = e / BZ, k1 = rest % L1, r = rest / L1;
         const int kx = kx0 + c, n2 = 2 * j + r;
         if (kx < H)
-            W[(size_t)(k1 * L2 + n2) * hs + kx] = mul(lds[(r * BZ + c) * LS + pidx(k1)], twid<TC, false>(tw, n2 * k1));
+            W[(size_t)(k1 * L2 + n2) * hs + kx] = mul(lds[(r * BZ + c) * LS + pidx(k1)], twid2<TC, L1, false>(w2, wb, n2 * k1));
     }
 }
 
 // ---- pass 7: z pass B (forward), FN, CN/AB2 ----------------------------------------------
-template <class TC, int M_>
-__global__ __launch_bounds__(kT) void k_zfwd_b(const TC *__restrict__ W, cd *__restrict__ om, cd *__restrict__ fn,
+template <class TC, int M_, int T>
+__global__ __launch_bounds__(T) void k_zfwd_b(const TC *__restrict__ W, cd *__restrict__ om, cd *__restrict__ fn,
                                                const TC *__restrict__ tw, int H, size_t hs, BigCnArgs a)
 {
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, BZ = bz_of<TC>(), LS = padded_len(L2);
     __shared__ TC lds[BZ * LS];
+    __shared__ TC w2[L2];
     const int N = 2 * H;
     const int tiles = (H + BZ - 1) / BZ;
     const int lb = xcd_order(blockIdx.x, gridDim.x);
     const int kx0 = (lb % tiles) * BZ, k1 = lb / tiles;
-    constexpr int CNT = BZ * L2, IT = (CNT + kT - 1) / kT;
+    constexpr int CNT = BZ * L2, IT = (CNT + T - 1) / T;
+    TabFetch<TC, L2, T> tf2;
+    tf2.fetch(tw, L1);
     TC wv[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         const int c = e % BZ, n2 = e / BZ;
         const int kx = kx0 + c;
         wv[i] = (e < CNT && kx < H) ? W[(size_t)(k1 * L2 + n2) * hs + kx] : TC{0, 0};
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * kT;
+        const int e = threadIdx.x + i * T;
         if (e < CNT) lds[(e % BZ) * LS + pidx(e / BZ)] = wv[i];
     }
+    tf2.put(w2);
     __syncthreads();
-    fft_lines<L2, LS, BZ, kT, false>(lds, tw, L1);
-    for (int e = threadIdx.x; e < BZ * L2; e += kT) {
+    fft_lines<L2, LS, BZ, T, false>(lds, w2, 1);
+    for (int e = threadIdx.x; e < BZ * L2; e += T) {
         const int c = e % BZ, k2 = e / BZ;
         const int kx = kx0 + c, kz = k1 + L1 * k2;
         const int iz = kz < H ? kz : (kz >= 2 * H ? kz - H : -1);
@@ -622,37 +701,37 @@ hipError_t launch_m(const BigReq &r)
     hipStream_t s = r.stream;
     switch (r.what) {
         case BIG_ZINV_A:
-            hipLaunchKernelGGL((k_zinv_a<TC, M_>), dim3(S::L2 * tiles * (unsigned)r.fields.pairs), dim3(kT), 0, s, r.om,
+            hipLaunchKernelGGL((k_zinv_a<TC, M_, kTZ>), dim3(S::L2 * tiles * (unsigned)r.fields.pairs), dim3(kTZ), 0, s, r.om,
                                static_cast<TC *>(r.b), tw, r.fields, H, r.hs);
             break;
         case BIG_ZINV_B:
-            hipLaunchKernelGGL((k_zinv_b<TC, M_>), dim3(S::L1 * tiles * (unsigned)r.fields.pairs), dim3(kT), 0, s,
+            hipLaunchKernelGGL((k_zinv_b<TC, M_, kTZ>), dim3(S::L1 * tiles * (unsigned)r.fields.pairs), dim3(kTZ), 0, s,
                                static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs);
             break;
         case BIG_XINV_A: {
             constexpr int NPT = (S::L2 / 2 - 1 + BZ - 1) / BZ;
-            hipLaunchKernelGGL((k_xinv_a<TC, M_>), dim3((NPT + 1) * M_ * (unsigned)r.planes), dim3(kT), 0, s,
+            hipLaunchKernelGGL((k_xinv_a<TC, M_, kTXA>), dim3((NPT + 1) * M_ * (unsigned)r.planes), dim3(kTXA), 0, s,
                                static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs, r.ms);
             break;
         }
         case BIG_XINV_B:
-            hipLaunchKernelGGL((k_xinv_b<TC, M_, 0>), dim3(M_ / 2, S::L1 / S::C4), dim3(kT), 0, s,
+            hipLaunchKernelGGL((k_xinv_b<TC, M_, 0, kTXB>), dim3(M_ / 2, S::L1 / S::C4), dim3(kTXB), 0, s,
                                static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, r.ms, nullptr, 0.0, 0.0);
             break;
         case BIG_XINV_CFL:
-            hipLaunchKernelGGL((k_xinv_b<TC, M_, 1>), dim3(M_ / 2, S::L1 / S::C4), dim3(kT), 0, s,
+            hipLaunchKernelGGL((k_xinv_b<TC, M_, 1, kTXB>), dim3(M_ / 2, S::L1 / S::C4), dim3(kTXB), 0, s,
                                static_cast<const TC *>(r.a), nullptr, tw, r.ms, r.cflrow, r.DX, r.DZ);
             break;
         case BIG_XFWD_B:
-            hipLaunchKernelGGL((k_xfwd_b<TC, M_>), dim3((M_ / 2) * (S::L2 / S::C5)), dim3(kT), 0, s,
+            hipLaunchKernelGGL((k_xfwd_b<TC, M_, kTXF>), dim3((M_ / 2) * (S::L2 / S::C5)), dim3(kTXF), 0, s,
                                static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.ms);
             break;
         case BIG_ZFWD_A:
-            hipLaunchKernelGGL((k_zfwd_a<TC, M_>), dim3((S::L2 / 2) * tiles), dim3(kT), 0, s,
+            hipLaunchKernelGGL((k_zfwd_a<TC, M_, kTZ>), dim3((S::L2 / 2) * tiles), dim3(kTZ), 0, s,
                                static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs, r.ms);
             break;
         case BIG_ZFWD_B:
-            hipLaunchKernelGGL((k_zfwd_b<TC, M_>), dim3(S::L1 * tiles), dim3(kT), 0, s, static_cast<const TC *>(r.a),
+            hipLaunchKernelGGL((k_zfwd_b<TC, M_, kTZ>), dim3(S::L1 * tiles), dim3(kTZ), 0, s, static_cast<const TC *>(r.a),
                                r.om, r.fn, tw, H, r.hs, r.cn);
             break;
         default:

@@ -5,7 +5,9 @@
 // from LDS into VGPRs (twiddled), the workgroup barriers, the thread runs the radix-R
 // DFT in registers and writes the R outputs to their Stockham (self-sorting)
 // positions, barrier.  The radix sequence is fixed at compile time (8, then 4, 2, 3,
-// 5, 7), so every loop below unrolls and the register arrays are sized exactly.
+// 5, 7; what remains of a length with a factor 5 or 7 goes through the prime-factor
+// radices 15, 14, 35, 21, 10 first, e.g. 210 = 15 x 14), so every loop below unrolls
+// and the register arrays are sized exactly.
 //
 // Convention: forward = e^{-2 pi i k n / L}, unnormalized; INV selects e^{+...}.
 // Twiddles come from one table tw[t] = e^{-2 pi i t / M} (t in [0, M)) shared by all
@@ -74,10 +76,17 @@ __host__ __device__ constexpr int pidx(int i) { return i + (i >> 3); }
 __host__ __device__ constexpr int padded_len(int L) { return L + (L + 7) / 8; }
 
 // ---- compile-time radix plan -------------------------------------------------
+// Lengths with a factor 5 or 7 take the composite radices first (fewer LDS passes and
+// barriers: 210 runs as 15 x 14 instead of 2 x 3 x 5 x 7; timed against 14 x 15 and
+// 10 x 21 on the 210-point lines of the 44100-point transform).
+__host__ __device__ constexpr int pick_radix57(int m)
+{
+    return (m % 15 == 0) ? 15 : (m % 14 == 0) ? 14 : (m % 35 == 0) ? 35 : (m % 21 == 0) ? 21 : (m % 10 == 0) ? 10 : 0;
+}
 __host__ __device__ constexpr int pick_radix(int m)
 {
-    return (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3
-         : (m % 5 == 0) ? 5 : (m % 7 == 0) ? 7 : m;
+    return (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (pick_radix57(m) != 0) ? pick_radix57(m) : (m % 2 == 0) ? 2
+         : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : (m % 7 == 0) ? 7 : m;
 }
 __host__ __device__ constexpr int stage_radix(int L, int s)
 {
@@ -109,7 +118,7 @@ __host__ __device__ constexpr bool smooth7(int L)
     int m = L;
     while (m > 1) {
         const int r = pick_radix(m);
-        if (r > 8) return false;
+        if (r > 8 && r != 10 && r != 14 && r != 15 && r != 21 && r != 35) return false;
         m /= r;
     }
     return true;
@@ -117,6 +126,9 @@ __host__ __device__ constexpr bool smooth7(int L)
 
 // ---- small DFTs in registers ----------------------------------------------------
 // Constants are the exact values cos/sin(2 pi q / R) to double precision.
+template <int R1, int R2, bool INV, class C>
+__device__ __forceinline__ void dft_pfa(C *v);
+
 template <int R, bool INV, class C>
 __device__ __forceinline__ void dft(C *v)
 {
@@ -199,7 +211,57 @@ __device__ __forceinline__ void dft(C *v)
         v[5] = sub(t2, u2);
         v[3] = add(t3, u3);
         v[4] = sub(t3, u3);
+    } else if constexpr (R == 10) {
+        dft_pfa<2, 5, INV, C>(v);
+    } else if constexpr (R == 14) {
+        dft_pfa<2, 7, INV, C>(v);
+    } else if constexpr (R == 15) {
+        dft_pfa<3, 5, INV, C>(v);
+    } else if constexpr (R == 21) {
+        dft_pfa<3, 7, INV, C>(v);
+    } else if constexpr (R == 35) {
+        dft_pfa<5, 7, INV, C>(v);
+    } else {
+        static_assert(R == 2, "radix without a register DFT");
     }
+}
+
+constexpr int inv_mod(int a, int m)
+{
+    for (int x = 1; x < m; ++x)
+        if ((a * x) % m == 1) return x;
+    return 0;
+}
+
+// Prime-factor (Good-Thomas) DFT of length R1 R2, gcd(R1, R2) = 1, without twiddles:
+// input n = (n1 R2 + n2 R1) mod R, output k = (k1 R2 (R2^-1 mod R1) + k2 R1 (R1^-1 mod R2))
+// mod R, and X[k] = sum_{n1} W_R1^{n1 k1} sum_{n2} W_R2^{n2 k2} x[n].  Every index is a
+// compile-time constant, so the permutations cost nothing.
+template <int R1, int R2, bool INV, class C>
+__device__ __forceinline__ void dft_pfa(C *v)
+{
+    constexpr int R = R1 * R2;
+    constexpr int E1 = R2 * inv_mod(R2 % R1, R1), E2 = R1 * inv_mod(R1 % R2, R2);
+    C t[R1][R2];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1)
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) t[n1][n2] = v[(n1 * R2 + n2 * R1) % R];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+        C c[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) c[n1] = t[n1][n2];
+        dft<R1, INV, C>(c);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) t[k1][n2] = c[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) dft<R2, INV, C>(t[k1]);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) v[(k1 * E1 + k2 * E2) % R] = t[k1][k2];
 }
 
 // ---- one Stockham stage over NL lines in LDS -----------------------------------
@@ -224,12 +286,19 @@ __device__ __forceinline__ void stage(C *buf, const C *__restrict__ tw, int twmu
             for (int r = 0; r < R; ++r) v[q][r] = src[pidx(j + r * LR)];
             if (NS > 1) {
                 // w^r for r = 1..R-1 from one table load: w^2 = w w, w^(2m) = (w^m)^2,
-                // w^(2m+1) = w^(2m) w (at most 3 roundings deep for R <= 8)
+                // w^(2m+1) = w^(2m) w (at most 3 roundings deep for r < 8); radices above
+                // 8 load w^8 too and take w^r = w^8 w^(r-8) for r >= 8 (4 roundings deep)
                 const C w1 = tw[opaque(k * TWS * twmul)];
                 C wp[R];
                 wp[1] = w1;
 #pragma unroll
-                for (int r = 2; r < R; ++r) wp[r] = (r % 2 == 0) ? mul(wp[r / 2], wp[r / 2]) : mul(wp[r - 1], w1);
+                for (int r = 2; r < R && r < 8; ++r) wp[r] = (r % 2 == 0) ? mul(wp[r / 2], wp[r / 2]) : mul(wp[r - 1], w1);
+                if constexpr (R > 8) {
+                    const C w8 = tw[opaque(8 * k * TWS * twmul)];
+                    wp[8] = w8;
+#pragma unroll
+                    for (int r = 9; r < R; ++r) wp[r] = mul(w8, wp[r - 8]);
+                }
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[q][r] = INV ? mulc(v[q][r], wp[r]) : mul(v[q][r], wp[r]);
             }
@@ -266,6 +335,7 @@ __device__ __forceinline__ void stages_from(C *buf, const C *__restrict__ tw, in
 // buf[l * LS + pidx(i)]; LS >= padded_len(L).  Callers barrier before (data
 // written) -- the first stage reads after the caller's __syncthreads -- and the
 // engine ends with a barrier, so results are visible to every thread on return.
+// tw may point to global memory or to an LDS table.
 template <int L, int LS, int NL, int T, bool INV, class C>
 __device__ __forceinline__ void fft_lines(C *buf, const C *__restrict__ tw, int twmul)
 {
