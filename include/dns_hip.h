@@ -121,6 +121,12 @@ dns_status dns_next_dt(dns_handle *h);
 /* E = sum m |om2|^2 / K^2, Z = sum m |om2|^2 (m = 1 for kx = 0, else 2); the
  * reference energy convention Q2 (pao_km3_cuda.cu:366-378).  Synchronizes. */
 dns_status dns_energy_enstrophy(dns_handle *h, double *E, double *Z);
+/* computePalOverEnsKmax2 (cuda/mem_cuda.cu:872-935), the input of the driver's ADAPT_VISC
+ * controller: sum w K^2 |om|^2 / ((sum w |om|^2 - |om(0,0)|^2) max K^2) with w = 1 on the
+ * kx = 0 and kx = N/2 - 1 columns and 2 elsewhere; 0 when the enstrophy sum is not
+ * positive.  Reduced on the device (the reference copies om2 to the host). */
+dns_status dns_pal_over_ens_kmax2(dns_handle *h, double *out);
+
 /* OM2PHYS + computeDisplaySigma (mem_cuda.cu:820-870): std-dev of the 8-bit
  * quantized physical vorticity.  Synchronizes. */
 dns_status dns_display_sigma(dns_handle *h, double *sigma);

@@ -81,6 +81,7 @@ SIGNATURES = {
     "dns_next_dt": (ctypes.c_int, [_P]),
     "dns_energy_enstrophy": (ctypes.c_int, [_P, _D, _D]),
     "dns_display_sigma": (ctypes.c_int, [_P, _D]),
+    "dns_pal_over_ens_kmax2": (ctypes.c_int, [_P, _D]),
     "dns_fetch_field": (ctypes.c_int, [_P, ctypes.c_int, _D]),
     "dns_get_scalars": (ctypes.c_int, [_P, ctypes.POINTER(Scalars)]),
     "dns_set_scalars": (ctypes.c_int, [_P, ctypes.POINTER(Scalars)]),
@@ -261,6 +262,12 @@ class Solver:
         e, z = ctypes.c_double(), ctypes.c_double()
         self._check(load_library().dns_energy_enstrophy(self._h, ctypes.byref(e), ctypes.byref(z)))
         return e.value, z.value
+
+    def pal_over_ens_kmax2(self) -> float:
+        """Palinstrophy / (enstrophy * max K^2), the ADAPT_VISC input (mem_cuda.cu:872-935)."""
+        v = ctypes.c_double()
+        self._check(load_library().dns_pal_over_ens_kmax2(self._h, ctypes.byref(v)))
+        return v.value
 
     def display_sigma(self) -> float:
         s = ctypes.c_double()

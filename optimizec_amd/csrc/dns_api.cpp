@@ -867,6 +867,27 @@ extern "C" dns_status dns_energy_enstrophy(dns_handle *h, double *E, double *Z)
     return DNS_OK;
 }
 
+// computePalOverEnsKmax2 (mem_cuda.cu:872-935) on the device: per-line partial sums, then
+// the ratio on the host; kmax2 = (H - 1)^2 + H^2, the largest K^2 of the stored modes
+extern "C" dns_status dns_pal_over_ens_kmax2(dns_handle *h, double *out)
+{
+    if (!h || !out) return DNS_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    const int parts = h->big ? h->N : h->H;
+    HIPCHK(h, dns_launch_palin(h->om, h->N, h->big ? 1 : 0, h->partEZ, h->partEZ + parts, h->stream));
+    std::vector<double> p(2 * (size_t)parts);
+    HIPCHK(h, hipMemcpyAsync(p.data(), h->partEZ, sizeof(double) * p.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double total = 0.0, pal = 0.0;
+    for (int i = 0; i < parts; ++i) {
+        total += p[i];
+        pal += p[parts + i];
+    }
+    const double kmax2 = double(h->H - 1) * double(h->H - 1) + double(h->H) * double(h->H);
+    *out = (total <= 0.0) ? 0.0 : pal / (total * kmax2);
+    return DNS_OK;
+}
+
 extern "C" dns_status dns_display_sigma(dns_handle *h, double *sigma)
 {
     if (!h || !sigma) return DNS_E_ARG;

@@ -484,6 +484,36 @@ __global__ __launch_bounds__(256) void k_energy(const cd *__restrict__ om, int N
     }
 }
 
+// Palinstrophy and enstrophy partials with the ADAPT_VISC controller's weights
+// (mem_cuda.cu:872-935): w = 1 on the kx = 0 and kx = H - 1 columns, 2 elsewhere; the
+// (0, 0) mode is left out of the enstrophy sum.  Block b sums line b of the state:
+// column kx = b of [kx][iz] (rowmajor 0, fused layout) or row iz = b of [iz][kx]
+// (rowmajor 1, multi-pass layout).
+__global__ __launch_bounds__(256) void k_palin(const cd *__restrict__ om, int N, int rowmajor,
+                                               double *__restrict__ partT, double *__restrict__ partP)
+{
+    __shared__ double red[4];
+    const int H = N / 2, b = blockIdx.x;
+    const int len = rowmajor ? H : N;
+    double tot = 0.0, pal = 0.0;
+    for (int i = threadIdx.x; i < len; i += 256) {
+        const int kx = rowmajor ? i : b, iz = rowmajor ? b : i;
+        const cd o = om[(size_t)b * len + i];
+        const double w = (kx == 0 || kx == H - 1) ? 1.0 : 2.0;
+        const double gz = gamma_of(iz, N);
+        const double k2 = double(kx) * double(kx) + gz * gz;
+        const double p = o.x * o.x + o.y * o.y;
+        if (kx != 0 || iz != 0) tot += w * p;
+        pal += w * k2 * p;
+    }
+    const double st = block_sum<256>(tot, red);
+    const double sp = block_sum<256>(pal, red);
+    if (threadIdx.x == 0) {
+        partT[b] = st;
+        partP[b] = sp;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_minmax(const double *__restrict__ f, size_t n, double *__restrict__ out)
 {
     __shared__ double smin[256], smax[256];
@@ -729,6 +759,12 @@ hipError_t dns_kernels_launch(int H, const LaunchReq &r)
 hipError_t dns_launch_energy(const cd *om, int N, int H, double *partE, double *partZ, hipStream_t s)
 {
     hipLaunchKernelGGL(k_energy, dim3(H), dim3(256), 0, s, om, N, partE, partZ);
+    return hipGetLastError();
+}
+
+hipError_t dns_launch_palin(const cd *om, int N, int rowmajor, double *partT, double *partP, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_palin, dim3(rowmajor ? N : N / 2), dim3(256), 0, s, om, N, rowmajor, partT, partP);
     return hipGetLastError();
 }
 

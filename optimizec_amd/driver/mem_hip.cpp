@@ -28,6 +28,8 @@
 #include <string>
 #include <sys/stat.h>
 #include <thread>
+
+#include "adapt_visc.hpp"
 #include <vector>
 
 #include "../../include/dns_hip.h"
@@ -226,6 +228,8 @@ struct Run {
     double K0 = 0, Re = 0, CFL = 0;
     int start_iter = 0;
     std::vector<MetricsRow> rows;
+    bool adapt = false;           // ADAPT_VISC=1
+    adaptvisc::State av;          // its controller state (mem_cuda.cu:1808)
 };
 
 dns_scalars scalars(const Run &r)
@@ -243,12 +247,13 @@ bool check(dns_status st, const Run &r, const char *what)
 }
 
 // appendMetricsRow (mem_cuda.cu:530-567, 996-1010): one row per iteration
-void append_metrics(Run &r, double sigma, double seconds)
+// pal: the palinstrophy ratio (mem_cuda.cu:872-935) when ADAPT_VISC is on, else 0
+void append_metrics(Run &r, double pal, double sigma, double seconds)
 {
     const dns_scalars s = scalars(r);
     const int steps = s.it - r.start_iter;
-    MetricsRow m{r.N, (int)s.K0, s.Re, s.cflnum, s.visc, s.dt, s.it, 0, (int)sigma, s.t, seconds / 60.0,
-                 seconds > 0.0 ? steps / seconds : 0.0};
+    MetricsRow m{r.N, (int)s.K0, s.Re, s.cflnum, s.visc, s.dt, s.it, (int)(10000.0 * pal), (int)sigma, s.t,
+                 seconds / 60.0, seconds > 0.0 ? steps / seconds : 0.0};
     if (!r.rows.empty() && r.rows.back().STEPS == s.it)
         r.rows.back() = m;
     else
@@ -271,7 +276,7 @@ bool write_metrics(const std::vector<MetricsRow> &rows, const std::string &path)
     return true;
 }
 
-bool write_meta(const Run &r, const std::string &path, double sigma, double minutes, double fps)
+bool write_meta(const Run &r, const std::string &path, double sigma, double pal, double minutes, double fps)
 {
     FILE *fp = std::fopen(path.c_str(), "w");
     if (!fp) {
@@ -286,7 +291,7 @@ bool write_meta(const Run &r, const std::string &path, double sigma, double minu
     const dns_scalars s = scalars(r);
     std::fprintf(fp, "timestamp,N,K0,Re,CFL,visc,T,IT,Update,sigma,10K_pal_over_Zkmax2,minutes,FPS,backend\n");
     std::fprintf(fp, "%s,%d,%.9g,%.9g,%.9g,%.9g,%.9g,%d,%d,%d,%.9g,%.9g,%.9g,HIP\n", ts, r.N, s.K0, s.Re, s.cflnum,
-                 s.visc, s.t, s.it, r.UPDATE, (int)sigma, 0.0, minutes, fps);
+                 s.visc, s.t, s.it, r.UPDATE, (int)sigma, 10000.0 * pal, minutes, fps);
     std::fclose(fp);
     std::printf("[CSV] Wrote %s (metadata)\n", path.c_str());
     return true;
@@ -444,14 +449,18 @@ bool dump_outputs(Run &r, const std::string &folder, double seconds, double fps)
         if (!check(dns_fetch_field(r.h, it.field, plane.data()), r, "dns_fetch_field")) return false;
         if (!write_pgm(plane, r.M, join(folder, it.name), it.clip, 0.001, 0.999, it.comp)) return false;
     }
-    double sigma = 0.0;
+    // sigma and the palinstrophy ratio of the last metrics row, or computed when there is
+    // none (mem_cuda.cu:1419-1422)
+    double sigma = 0.0, pal = 0.0;
     if (r.rows.empty()) {
         if (!check(dns_display_sigma(r.h, &sigma), r, "dns_display_sigma")) return false;
+        if (!check(dns_pal_over_ens_kmax2(r.h, &pal), r, "dns_pal_over_ens_kmax2")) return false;
     } else {
         sigma = r.rows.back().SIG;
+        pal = r.rows.back().PALIN / 10000.0;
     }
     return write_spectrum(r, join(folder, "energy_spectrum.csv")) && write_metrics(r.rows, join(folder, "metrics.csv")) &&
-           write_meta(r, join(folder, "meta.csv"), sigma, seconds / 60.0, fps);
+           write_meta(r, join(folder, "meta.csv"), sigma, pal, seconds / 60.0, fps);
 }
 
 bool snapshot(Run &r, bool with_restart, double seconds, double fps)
@@ -468,9 +477,10 @@ bool snapshot(Run &r, bool with_restart, double seconds, double fps)
     } else {
         std::printf("[SIGNAL] Received SIGUSR1; saving current state and continuing.\n");
     }
-    double sigma = 0.0;
+    double sigma = 0.0, pal = 0.0;
     if (!check(dns_display_sigma(r.h, &sigma), r, "dns_display_sigma")) return false;
-    append_metrics(r, sigma, seconds);
+    if (r.adapt && !check(dns_pal_over_ens_kmax2(r.h, &pal), r, "dns_pal_over_ens_kmax2")) return false;
+    append_metrics(r, pal, sigma, seconds);
     return dump_outputs(r, folder, seconds, fps);
 }
 
@@ -545,8 +555,6 @@ int main(int argc, char **argv)
                 STEPS, CFL, UPDATE, ADAPT_VISC ? 1 : 0, MOV ? 1 : 0);
     if (phase_timing) std::printf("[PHASE] PHASE_TIMING enabled; kernel times are taken with HIP events.\n");
     if (!restart_path.empty()) std::printf(" restart=%s\n", restart_path.c_str());
-    if (ADAPT_VISC)
-        std::printf("[WARN] ADAPT_VISC=1 is not implemented by this build; the viscosity stays 1/Re.\n");
     if (MOV) std::printf("[WARN] MOV movie frames are not written by this build.\n");
 
     Run r;
@@ -556,6 +564,7 @@ int main(int argc, char **argv)
     r.K0 = K0;
     r.Re = Re;
     r.CFL = CFL;
+    r.adapt = ADAPT_VISC;
     dns_config cfg{N, Re, K0, CFL, DNS_PREC_FP64, 0, DNS_PATH_AUTO};
     dns_status st = dns_create(&cfg, &r.h);
     if (st != DNS_OK) {
@@ -631,12 +640,28 @@ int main(int argc, char **argv)
             return 1;
         }
         if (it == 1 || it % UPDATE == 0 || it == STEPS) {
-            double sigma = 0.0;
+            double sigma = 0.0, pal = 0.0;
             if (!check(dns_next_dt(r.h), r, "next_dt") || !check(dns_display_sigma(r.h, &sigma), r, "sigma")) {
                 dns_destroy(r.h);
                 return 1;
             }
-            append_metrics(r, sigma, elapsed(tbegin) - paused);
+            if (r.adapt) {  // mem_cuda.cu:1871-1875
+                if (!check(dns_pal_over_ens_kmax2(r.h, &pal), r, "dns_pal_over_ens_kmax2")) {
+                    dns_destroy(r.h);
+                    return 1;
+                }
+                dns_scalars sv = scalars(r);
+                double Re_new = 0.0, visc_new = 0.0;
+                if (adaptvisc::update(r.av, r.N, sv.K0, sv.visc, pal, &Re_new, &visc_new)) {
+                    sv.Re = Re_new;
+                    sv.visc = visc_new;
+                    if (!check(dns_set_scalars(r.h, &sv), r, "dns_set_scalars")) {
+                        dns_destroy(r.h);
+                        return 1;
+                    }
+                }
+            }
+            append_metrics(r, pal, sigma, elapsed(tbegin) - paused);
             const dns_scalars s = scalars(r);
             std::printf(" ITERATION %6d T=%12.10f DT=%10.8f CN=%10.8f CFLM=%.6f Re=%12.5e VISC=%12.5e PALIN=%d SIG=%d "
                         "FPS=%.3f\n",
@@ -722,12 +747,13 @@ int main(int argc, char **argv)
             return 128 + sig;
         }
         std::printf("[SIGNAL] Received %s (%d); dumping current state before exit.\n", signal_name(sig), sig);
-        double sigma = 0.0;
-        if (!check(dns_next_dt(r.h), r, "next_dt") || !check(dns_display_sigma(r.h, &sigma), r, "sigma")) {
+        double sigma = 0.0, pal = 0.0;
+        if (!check(dns_next_dt(r.h), r, "next_dt") || !check(dns_display_sigma(r.h, &sigma), r, "sigma") ||
+            (r.adapt && !check(dns_pal_over_ens_kmax2(r.h, &pal), r, "dns_pal_over_ens_kmax2"))) {
             dns_destroy(r.h);
             return 1;
         }
-        append_metrics(r, sigma, elap);
+        append_metrics(r, pal, sigma, elap);
         const bool ok = dump_outputs(r, output_folder(r.N, r.K0, r.Re, r.CFL, scalars(r).it), elap, fps);
         dns_destroy(r.h);
         return ok ? 128 + sig : 1;

@@ -32,6 +32,10 @@
  *                                    spectrum shape, determinism.
  *   or_display_sigma                 mem_cuda.cu:651-870.  Pinned: brute recomputation
  *                                    on small fields, KAT SIG at it=1.
+ *   or_pal_over_ens_kmax2            mem_cuda.cu:872-935.  Pinned: single-mode fields give
+ *                                    K^2 / kmax2 exactly; two modes on the weight-1 and
+ *                                    weight-2 columns give the hand-computed weighted mean;
+ *                                    the (0,0) mode leaves the ratio unchanged.
  */
 #include "dns_oracle.h"
 
@@ -1239,6 +1243,33 @@ void or_energy_enstrophy(const or_state *S, double *E, double *Z)
     }
     *E = e;
     *Z = z;
+}
+
+/* Palinstrophy over enstrophy, scaled by the largest K^2 (mem_cuda.cu:872-935, the
+ * ADAPT_VISC controller's input): sum w K^2 |om|^2 / ((sum w |om|^2 - |om(0,0)|^2) kmax2)
+ * with w = 1 on the ix = 0 and ix = H - 1 columns and 2 elsewhere (the reference's
+ * weights, :914-917), kmax2 = max K^2 over the stored modes, 0 when a denominator is not
+ * positive (:930-932). */
+double or_pal_over_ens_kmax2(const or_state *S)
+{
+    const int N = S->N, H = S->H;
+    double total_full = 0.0, pal = 0.0, kmax2 = 0.0, k00 = 0.0;
+    for (int iz = 0; iz < N; ++iz) {
+        const double gz = gam_of(iz, N);
+        for (int ix = 0; ix < H; ++ix) {
+            const cplx o = S->om2[(size_t)iz * H + ix];
+            const double w = (ix == 0 || ix == H - 1) ? 1.0 : 2.0;
+            const double k2 = (double)ix * ix + gz * gz;
+            const double p = o.re * o.re + o.im * o.im;
+            total_full += w * p;
+            pal += w * k2 * p;
+            if (k2 > kmax2) kmax2 = k2;
+            if (ix == 0 && iz == 0) k00 = p;
+        }
+    }
+    const double total = total_full - k00;
+    if (total <= 0.0 || kmax2 <= 0.0) return 0.0;
+    return pal / (total * kmax2);
 }
 
 /* OM2PHYS: om2 into its 3/2-grid rows of UC comp 2, inverse to UR comp 2

@@ -90,6 +90,9 @@ void   or_energy_enstrophy(const or_state *S, double *E, double *Z);
 double or_cflm(const or_state *S);              /* max |u|/DX + |w|/DZ on ur   */
 void   or_om2phys(or_state *S);                 /* om2 -> ur comp 2 (om2phys_cuda.cu:153-205) */
 double or_display_sigma(const or_state *S, int comp); /* mem_cuda.cu:820-870 */
+/* sum w K^2 |om|^2 / ((sum w |om|^2 - |om(0,0)|^2) max K^2): the ADAPT_VISC input
+ * (mem_cuda.cu:872-935) */
+double or_pal_over_ens_kmax2(const or_state *S);
 /* FN of the current state (advective form) into fn [N][H]. */
 void   or_nonlinear_adv(or_state *S, double *fn);
 /* FN of the current state (uiuj form, from om2) into fn [N][H]. */

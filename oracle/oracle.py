@@ -76,6 +76,8 @@ def _declare(L):
     L.or_cflm.restype = ctypes.c_double
     L.or_display_sigma.argtypes = [_P, ctypes.c_int]
     L.or_display_sigma.restype = ctypes.c_double
+    L.or_pal_over_ens_kmax2.argtypes = [_P]
+    L.or_pal_over_ens_kmax2.restype = ctypes.c_double
     L.or_nonlinear_adv.argtypes = [_P, _D]
     L.or_nonlinear_uiuj.argtypes = [_P, _D]
     for name in ("or_get_om2", "or_set_om2", "or_get_fnm1", "or_set_fnm1"):
@@ -192,6 +194,9 @@ class Oracle:
 
     def display_sigma(self, comp: int = 2) -> float:
         return lib().or_display_sigma(self._h, comp)
+
+    def pal_over_ens_kmax2(self) -> float:
+        return lib().or_pal_over_ens_kmax2(self._h)
 
     def nonlinear(self, form: int = 1) -> np.ndarray:
         out = np.empty((self.N, self.H), dtype=np.complex128)

@@ -126,6 +126,14 @@ dns_status dns_display_sigma(dns_handle *h, double *sigma)
     return DNS_OK;
 }
 
+/* toy palinstrophy ratio: 10000 x ratio = 30, above the controller's target of 20 */
+dns_status dns_pal_over_ens_kmax2(dns_handle *h, double *out)
+{
+    (void)h;
+    *out = 0.003;
+    return DNS_OK;
+}
+
 dns_status dns_fetch_field(dns_handle *h, int field, double *host)
 {
     const int M = h->sc.M;

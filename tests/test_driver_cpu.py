@@ -194,3 +194,36 @@ def test_trailing_restart_argument(rundir):
     # a restart file of another N is rejected with exit code 1
     r = run(rundir, ["32", "1000", "4", "60", "0.1", "5", "0", f"{folder}/restart.bin"])
     assert r.returncode == 1
+
+
+def test_adapt_visc_controller_and_palin_columns(rundir):
+    """ADAPT_VISC=1: on every diagnostic step the driver feeds the backend's palinstrophy
+    ratio (0.003 in the mock: 10000 x ratio = 30, above the target 20) to the controller
+    (mem_cuda.cu:492-528), whose correction is clamped to +1%, so VISC grows by exactly
+    exp(0.01) per diagnostic step; PALIN = 30 in the ITERATION lines and metrics.csv, and
+    meta.csv's 10K_pal_over_Zkmax2 column carries it."""
+    import math
+
+    p = start(rundir, ["16", "1000", "4", "0", "0.1", "5", "1"], {"MOCK_STEP_MS": "5"})
+    lines = wait_for_iteration(p, 20)
+    p.send_signal(signal.SIGTERM)
+    out, _ = p.communicate(timeout=30)
+    assert p.returncode == 128 + signal.SIGTERM
+    import re
+
+    its = [dict(re.findall(r"(\w+)=\s*(\S+)", ln)) for ln in lines + out.splitlines(True)
+           if ln.startswith(" ITERATION")]
+    visc = [float(kv["VISC"]) for kv in its]
+    palin = [int(kv["PALIN"]) for kv in its]
+    assert all(pv == 30 for pv in palin)
+    assert visc[0] == pytest.approx(1e-3 * math.exp(0.01), rel=1e-5)  # the first diagnostic step
+    for a, b in zip(visc, visc[1:]):
+        assert b / a == pytest.approx(math.exp(0.01), rel=1e-4)
+    folder = [d for d in os.listdir(rundir) if d.startswith("output_16_4_1E03_0.10_")][0]
+    with open(rundir / folder / "metrics.csv") as f:
+        rows = [ln.strip().split(",") for ln in f][1:]
+    assert rows and all(r[7] == "30" for r in rows)
+    with open(rundir / folder / "meta.csv") as f:
+        f.readline()
+        meta = f.readline().strip().split(",")
+    assert float(meta[10]) == pytest.approx(30.0)
