@@ -82,3 +82,70 @@ def test_driver_restart_roundtrip(tmp_path):
     final1 = [ln for ln in r1.stdout.splitlines() if ln.startswith(" Final T=")][0]
     final2 = [ln for ln in r2.stdout.splitlines() if ln.startswith(" Final T=")][0]
     assert final1 == final2
+
+
+def test_driver_output_files_match_oracle(oracle_mod, tmp_path):
+    """SIGTERM dump of a real run (N=64, k4exp seed 2): the five PGMs and
+    energy_spectrum.csv against the oracle's restatement of the reference writers
+    (oracle/outputs.py) applied to the oracle's fields at the same iteration."""
+    import numpy as np
+
+    import outputs  # oracle/ (test infrastructure)
+
+    N, M = 64, 96
+    print("mem_hip output files vs oracle writers, N=64", flush=True)
+    env = dict(os.environ, DNS_IC="k4exp", DNS_SEED="2")
+    p = subprocess.Popen([DRIVER, str(N), "1e4", "4", "0", "0.1", "5", "0"], cwd=tmp_path, env=env,
+                         stdout=subprocess.PIPE, text=True)
+    t_end = time.time() + 60
+    while time.time() < t_end:
+        ln = p.stdout.readline()
+        if ln.startswith(" ITERATION") and int(ln.split()[1]) >= 10:
+            break
+    p.send_signal(signal.SIGTERM)
+    p.communicate(timeout=60)
+    assert p.returncode == 128 + signal.SIGTERM
+    folder = [d for d in os.listdir(tmp_path) if d.startswith("output_64_")][0]
+    it = int(folder.rsplit("_", 1)[1])
+    o = oracle_mod.Oracle(N, 1e4, 4.0, 0.1)
+    o.init_config(2)
+    o.start(form=1)
+    o.run(it, 5, form=1)
+    o.velocity_from_om2()
+    u, w = o.ur(0).copy(), o.ur(1).copy()
+    o.om2phys()
+    om_phys = o.ur(2).copy()
+    om = o.om2()
+    kx = np.arange(N // 2)[None, :]
+    kz = np.where(np.arange(N) < N // 2, np.arange(N), np.arange(N) - N)[:, None]
+    k2 = (kx ** 2 + kz ** 2).astype(float)
+    g = np.zeros((M, 3 * N // 4 + 1), dtype=np.complex128)
+    psi = np.where(k2 > 0, om / np.where(k2 > 0, k2, 1.0), 0.0)
+    g[: N // 2, : N // 2] = psi[: N // 2]
+    g[N:, : N // 2] = psi[N // 2:]
+    fields = {"u_velocity.pgm": (u, False), "v_velocity.pgm": (w, False),
+              "kinetic.pgm": (np.hypot(u, w), False), "omega.pgm": (om_phys, True),
+              "stream.pgm": (oracle_mod.c2r_2d(g), False)}
+    for name, (f, clip) in fields.items():
+        with open(tmp_path / folder / name, "rb") as fh:
+            data = fh.read()
+        head = f"P5\n{M} {M}\n255\n".encode()
+        assert data.startswith(head)
+        got = np.frombuffer(data[len(head):], dtype=np.uint8).reshape(M, M).astype(int)
+        want = outputs.pgm_pixels(f, clipped=clip).astype(int)
+        diff = np.abs(got - want)
+        assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3, (name, diff.max(), np.mean(diff > 0))
+    with open(tmp_path / folder / "energy_spectrum.csv") as fh:
+        fh.readline()
+        got = [tuple(float(x) for x in ln.split(",")) for ln in fh if ln.strip()]
+    want = outputs.energy_spectrum(u, w)
+    total = sum(e for _, e, _ in want)
+    # bins below 1e-10 of the total hold the R2C's roundoff (oracle) or the far spectral
+    # tail (both); compare the rest, to 1e-9 where the shell carries 1e-6 of the energy
+    got = [r for r in got if r[1] > 1e-10 * total]
+    want = [r for r in want if r[1] > 1e-10 * total]
+    assert len(got) == len(want) and len(want) > 20
+    for (rg, eg, cg), (rw, ew, cw) in zip(got, want):
+        assert rg == rw and int(cg) == cw
+        assert abs(eg / ew - 1) < (1e-9 if ew > 1e-6 * total else 1e-3), (rg, eg, ew)
+    o.close()
