@@ -10,7 +10,9 @@
 //
 // Environment: PHASE_TIMING, BENCH_WARMUP_STEPS, RESTART_STEPS, MOV, MOV_DIR (as in
 // mem_cuda.cu:232-243, 1680-1806), plus DNS_IC=pao|k4exp (initial condition; the
-// reference's PAO-k^-3 by default) and DNS_SEED (seed of the k4exp spectrum).
+// reference's PAO-k^-3 by default) and DNS_SEED (seed of the k4exp spectrum).  MOV frames
+// go to $MOV_DIR/hip_<N>_<K0>_<Re>_<CFL>_U<UPDATE>/ (MOV_DIR defaults to ./movies; the
+// reference's default is a home directory of its author and its prefix "cuda_").
 //
 // Signals (mem_cuda.cu:77-135, 1923-2081): USR1 snapshot + continue; RTMIN snapshot +
 // restart.bin + continue; USR2 pause until CONT; HUP/TERM save + exit 128+sig; INT exit
@@ -30,6 +32,9 @@
 #include <thread>
 
 #include "adapt_visc.hpp"
+#include "movie.hpp"
+
+#include <dirent.h>
 #include <vector>
 
 #include "../../include/dns_hip.h"
@@ -130,6 +135,52 @@ bool make_dir(const std::string &path)
 }
 
 std::string join(const std::string &folder, const char *name) { return folder + "/" + name; }
+
+// mkdir -p
+bool make_dirs(const std::string &path)
+{
+    for (size_t i = 1; i <= path.size(); ++i)
+        if (i == path.size() || path[i] == '/') {
+            const std::string p = path.substr(0, i);
+            if (mkdir(p.c_str(), 0775) != 0 && errno != EEXIST) {
+                std::fprintf(stderr, "Failed to create folder %s: %s\n", p.c_str(), std::strerror(errno));
+                return false;
+            }
+        }
+    return true;
+}
+
+// makeMovieFolderName (mem_cuda.cu:266-282)
+std::string movie_folder(int N, double K0, double Re, double CFL, int update)
+{
+    const char *root = std::getenv("MOV_DIR");
+    char cfl[64];
+    std::snprintf(cfl, sizeof cfl, "%.3f", CFL);
+    const std::string name = "hip_" + std::to_string(N) + "_" + std::to_string((int)K0) + "_" + sci_no_plus(Re) + "_" +
+                             cfl + "_U" + std::to_string(update);
+    return join(root && root[0] ? root : "movies", name.c_str());
+}
+
+// findLastMovieFrameIndex (mem_cuda.cu:291-332): the largest NNNNN of omega_inferno_NNNNN.png
+int last_movie_frame(const std::string &folder)
+{
+    int last = 0;
+    DIR *d = opendir(folder.c_str());
+    if (!d) return 0;
+    const std::string pre = "omega_inferno_", suf = ".png";
+    while (dirent *e = readdir(d)) {
+        const std::string n = e->d_name;
+        if (n.size() <= pre.size() + suf.size() || n.compare(0, pre.size(), pre) != 0 ||
+            n.compare(n.size() - suf.size(), suf.size(), suf) != 0)
+            continue;
+        const std::string digits = n.substr(pre.size(), n.size() - pre.size() - suf.size());
+        if (!std::all_of(digits.begin(), digits.end(), [](char c) { return std::isdigit((unsigned char)c) != 0; }))
+            continue;
+        last = std::max(last, std::atoi(digits.c_str()));
+    }
+    closedir(d);
+    return last;
+}
 
 std::string output_folder(int N, double K0, double Re, double CFL, int it)
 {
@@ -555,7 +606,6 @@ int main(int argc, char **argv)
                 STEPS, CFL, UPDATE, ADAPT_VISC ? 1 : 0, MOV ? 1 : 0);
     if (phase_timing) std::printf("[PHASE] PHASE_TIMING enabled; kernel times are taken with HIP events.\n");
     if (!restart_path.empty()) std::printf(" restart=%s\n", restart_path.c_str());
-    if (MOV) std::printf("[WARN] MOV movie frames are not written by this build.\n");
 
     Run r;
     r.N = N;
@@ -619,6 +669,20 @@ int main(int argc, char **argv)
     const char *ew = std::getenv("BENCH_WARMUP_STEPS");
     if (ew && ew[0]) warmup = std::max(0, std::atoi(ew));
     r.start_iter = scalars(r).it;
+    // movie frames (mem_cuda.cu:1834-1844)
+    std::string mov_folder;
+    int mov_index = 0;
+    const int mov_f = movie::scale_f(N);
+    if (MOV) {
+        mov_folder = movie_folder(N, K0, Re, CFL, UPDATE);
+        if (!make_dirs(mov_folder)) {
+            dns_destroy(r.h);
+            return 1;
+        }
+        mov_index = last_movie_frame(mov_folder);
+        std::printf("[MOV] Saving omega_inferno_%%05d.png every %d steps to %s (F=%d, next=%05d)\n", UPDATE,
+                    mov_folder.c_str(), mov_f, mov_index + 1);
+    }
     bool interrupted = false, steady_started = false;
     double paused = 0.0, steady_paused0 = 0.0;
     int steady_iter0 = r.start_iter;
@@ -667,6 +731,27 @@ int main(int argc, char **argv)
                         "FPS=%.3f\n",
                         it, s.t, s.dt, s.cn, s.cflm, s.Re, s.visc, r.rows.back().PALIN, r.rows.back().SIG,
                         r.rows.back().FPS);
+            if (MOV && it > 0 && it % UPDATE == 0) {  // mem_cuda.cu:1899-1910: frame of omega
+                std::vector<double> plane((size_t)r.M * r.M);
+                if (!check(dns_fetch_field(r.h, DNS_FIELD_OMEGA, plane.data()), r, "dns_fetch_field")) {
+                    dns_destroy(r.h);
+                    return 1;
+                }
+                int fw = 0, fh = 0;
+                double fsig = 0.0;
+                const std::vector<unsigned char> rgb = movie::render(plane, r.M, mov_f, &fw, &fh, &fsig);
+                char name[64];
+                std::snprintf(name, sizeof name, "omega_inferno_%05d.png", mov_index + 1);
+                const std::string path = join(mov_folder, name);
+                if (!movie::write_png(path, fw, fh, rgb)) {
+                    std::fprintf(stderr, "dnsCudaSaveFieldInfernoPng: failed to write %s\n", path.c_str());
+                    dns_destroy(r.h);
+                    return 1;
+                }
+                std::printf("[MOV] Wrote %s (Inferno PNG, %dx%d, F=%d, comp=2, sigma=%d)\n", path.c_str(), fw, fh, mov_f,
+                            (int)fsig);
+                ++mov_index;
+            }
             std::fflush(stdout);
             if (!steady_started && it >= warmup) {
                 dns_synchronize(r.h);

@@ -78,3 +78,76 @@ def energy_spectrum(u, w):
             r0, r1 = rmax * i / nbins, rmax * (i + 1) / nbins
             rows.append((0.5 * (r0 + r1), float(esum[i]), int(cnt[i])))
     return rows
+
+
+# ---- MOV frames (color_cuda.cu:19-597, mem_cuda.cu:245-332, 1834-1910) ----------------
+def inferno_lut():
+    """256 x 3 bytes: linear ramp through the stops (0, (0,0,4)), (.25, (87,15,109)),
+    (.5, (187,55,84)), (.75, (249,142,8)), (1, (252,255,164)), evaluated in float32 and
+    truncated, as the reference builds it (color_cuda.cu:40-70)."""
+    stops = [(0.00, (0, 0, 4)), (0.25, (87, 15, 109)), (0.50, (187, 55, 84)), (0.75, (249, 142, 8)),
+             (1.00, (252, 255, 164))]
+    f32 = np.float32
+    lut = np.zeros((256, 3), dtype=np.uint8)
+    for i in range(256):
+        x = f32(i) / f32(255.0)
+        s = 0
+        while s + 1 < 5 and x > f32(stops[s + 1][0]):
+            s += 1
+        (pa, ca), (pb, cb) = stops[s], stops[min(s + 1, 4)]
+        span = max(f32(1e-12), f32(pb) - f32(pa))
+        t = min(f32(1.0), max(f32(0.0), (x - f32(pa)) / span))
+        for k in range(3):
+            lut[i, k] = int((f32(1.0) - t) * f32(ca[k]) + t * f32(cb[k]))
+    return lut
+
+
+def movie_scale_f(N):
+    """dnsCudaMovieScaleF (color_cuda.cu:486-493)"""
+    return {9216: 8, 16384: 12, 29400: 14}.get(N, max(1, (3 * N // 2 + 3839) // 3840))
+
+
+def movie_frame(f, F):
+    """RGB frame of an M x M plane [z][x] sampled every F points (renderInfernoKernel,
+    color_cuda.cu:185-233), in double; returns (rgb [h][w][3], grey-level sigma)."""
+    f = np.asarray(f, dtype=np.float64)
+    M = f.shape[0]
+    fin = np.isfinite(f)
+    mn, mx = (float(f[fin].min()), float(f[fin].max())) if fin.any() else (0.0, 0.0)
+    rng = mx - mn
+    const = not rng > 1e-12
+    inv = 1.0 if const else 1.0 / rng
+    grey = np.where(fin & (not const), np.clip(1.0 + (f - mn) * inv * 254.0, 1.0, 255.0), 128.0)
+    grey = np.floor(grey).astype(np.int64)
+    mean = grey.mean()
+    sigma = math.sqrt(max(0.0, (grey.astype(np.float64) ** 2).mean() - mean * mean))
+    idx = np.minimum(np.arange(0, (M + F - 1) // F) * F, M - 1)
+    pix = grey[np.ix_(idx, idx)]
+    lo, hi = mean - 2.5 * sigma, mean + 2.5 * sigma
+    if sigma >= 1.0 and hi > lo:
+        pix = np.clip(np.floor((pix - lo) * 255.0 / (hi - lo) + 0.5), 0, 255).astype(np.int64)
+    return inferno_lut()[pix], sigma
+
+
+def read_png_rgb(data):
+    """(width, height, rgb) of an 8-bit RGB PNG with filter 0 rows; checks every CRC."""
+    import struct
+    import zlib
+
+    assert data[:8] == bytes([137, 80, 78, 71, 13, 10, 26, 10])
+    pos, idat, w = 8, b"", None
+    while pos < len(data):
+        n = struct.unpack(">I", data[pos:pos + 4])[0]
+        typ = data[pos + 4:pos + 8]
+        body = data[pos + 8:pos + 8 + n]
+        crc = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])[0]
+        assert zlib.crc32(typ + body) & 0xFFFFFFFF == crc, typ
+        if typ == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", body[:10])
+            assert (depth, ctype) == (8, 2)
+        elif typ == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), dtype=np.uint8).reshape(h, 3 * w + 1)
+    assert np.all(raw[:, 0] == 0)
+    return w, h, raw[:, 1:].reshape(h, w, 3)

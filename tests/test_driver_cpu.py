@@ -227,3 +227,31 @@ def test_adapt_visc_controller_and_palin_columns(rundir):
         f.readline()
         meta = f.readline().strip().split(",")
     assert float(meta[10]) == pytest.approx(30.0)
+
+
+def test_mov_frames(rundir):
+    """MOV: a PNG of omega on every diagnostic step that is a multiple of UPDATE, numbered on
+    from the last frame already in $MOV_DIR/hip_<N>_<K0>_<Re>_<CFL>_U<UPDATE>/
+    (mem_cuda.cu:266-332, 1834-1910); the images decode (CRCs, zlib) and match the oracle's
+    restatement of the frame renderer on the backend's field."""
+    import numpy as np
+
+    import outputs  # oracle/ (test infrastructure)
+
+    movdir = rundir / "mov"
+    folder = movdir / "hip_16_4_1E03_0.100_U5"
+    folder.mkdir(parents=True)
+    (folder / "omega_inferno_00007.png").write_bytes(b"")
+    r = run(rundir, ["16", "1000", "4", "12", "0.1", "5", "0", "MOV"], {"MOV_DIR": str(movdir)})
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"[MOV] Saving omega_inferno_%05d.png every 5 steps to {folder} (F=1, next=00008)" in r.stdout
+    names = sorted(p.name for p in folder.iterdir())
+    assert names == ["omega_inferno_00007.png", "omega_inferno_00008.png", "omega_inferno_00009.png"]
+    M = 24
+    x = np.arange(M)
+    field = np.sin(0.05 * x[None, :] * 3) * np.cos(0.03 * x[:, None])  # the mock's omega, up to a positive factor
+    want, _ = outputs.movie_frame(field, 1)
+    for n in ("omega_inferno_00008.png", "omega_inferno_00009.png"):
+        w, h, rgb = outputs.read_png_rgb((folder / n).read_bytes())
+        assert (w, h) == (M, M)
+        assert np.mean(np.all(rgb == want, axis=2)) > 0.97

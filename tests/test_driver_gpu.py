@@ -149,3 +149,33 @@ def test_driver_output_files_match_oracle(oracle_mod, tmp_path):
         assert rg == rw and int(cg) == cw
         assert abs(eg / ew - 1) < (1e-9 if ew > 1e-6 * total else 1e-3), (rg, eg, ew)
     o.close()
+
+
+def test_driver_mov_frames_match_oracle(oracle_mod, tmp_path):
+    """MOV frames of a real run (N=64, k4exp seed 4, UPDATE=5): the frame of step 10
+    against the oracle's frame renderer on the oracle's omega at step 10."""
+    import numpy as np
+
+    import outputs  # oracle/ (test infrastructure)
+
+    N, M = 64, 96
+    print("mem_hip MOV frames vs oracle, N=64", flush=True)
+    env = dict(os.environ, DNS_IC="k4exp", DNS_SEED="4", MOV_DIR=str(tmp_path / "mov"))
+    r = subprocess.run([DRIVER, str(N), "1e4", "4", "10", "0.1", "5", "0", "MOV"], cwd=tmp_path, env=env,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    folder = tmp_path / "mov" / "hip_64_4_1E04_0.100_U5"
+    assert sorted(p.name for p in folder.iterdir()) == ["omega_inferno_00001.png", "omega_inferno_00002.png"]
+    o = oracle_mod.Oracle(N, 1e4, 4.0, 0.1)
+    o.init_config(4)
+    o.start(form=1)
+    o.run(10, 5, form=1)
+    o.om2phys()
+    want, sigma = outputs.movie_frame(o.ur(2), outputs.movie_scale_f(N))
+    w, h, rgb = outputs.read_png_rgb((folder / "omega_inferno_00002.png").read_bytes())
+    assert (w, h) == (M, M)
+    same = np.mean(np.all(rgb == want, axis=2))
+    print(f"sigma {sigma:.2f}, identical pixels {same:.5f}")
+    assert same > 0.999
+    assert f"(Inferno PNG, {M}x{M}, F=1, comp=2, sigma={int(sigma)})" in r.stdout
+    o.close()

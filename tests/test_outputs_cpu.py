@@ -57,3 +57,25 @@ def test_energy_spectrum_single_modes():
     # the counts of all bins add up to every mode of the half plane but (0, 0)
     rows = outputs.energy_spectrum(np.random.default_rng(1).standard_normal((M, M)), np.zeros((M, M)))
     assert sum(c for _, _, c in rows) == M * M - 1
+
+
+def test_inferno_lut_and_frames():
+    lut = outputs.inferno_lut()
+    assert tuple(lut[0]) == (0, 0, 4) and tuple(lut[255]) == (252, 255, 164)
+    # i = 128: x = 128/255 in the (0.5, 0.75] segment, t = (x - 0.5) / 0.25 = 0.00784
+    assert tuple(lut[128]) == (187, 55, 83)
+    # i = 64: x = 0.25098, just past the 0.25 stop, t = 0.0039: (87.4, 15.2, 108.9)
+    assert tuple(lut[64]) == (87, 15, 108)
+    assert outputs.movie_scale_f(4096) == 2 and outputs.movie_scale_f(512) == 1
+    assert outputs.movie_scale_f(29400) == 14
+    # a constant plane is grey 128 everywhere, sigma 0 -> colour index 128
+    rgb, sigma = outputs.movie_frame(np.full((12, 12), 2.0), 1)
+    assert sigma == 0.0 and np.all(rgb == lut[128])
+    # two-valued plane: grey 1 and 255 in equal numbers, sigma 127 -> mean +- 2.5 sigma
+    # covers [-190.5, 445.5]: grey 1 -> index floor((1 + 190.5) 255 / 635 + 0.5) = 77,
+    # grey 255 -> floor((255 + 190.5) 255 / 635 + 0.5) = 179
+    f = np.zeros((8, 8))
+    f[:, 4:] = 1.0
+    rgb, sigma = outputs.movie_frame(f, 2)
+    assert sigma == 127.0 and rgb.shape == (4, 4, 3)
+    assert np.all(rgb[:, :2] == lut[77]) and np.all(rgb[:, 2:] == lut[179])
