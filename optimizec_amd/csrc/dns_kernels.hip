@@ -274,6 +274,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
     static_assert(T % 64 == 0, "whole wavefronts");
     static_assert(H % T == 0 || H < T, "pair slots must tile the half row");
     __shared__ TC lds[3 * LS];
+    __shared__ TC twl[H];  // W_H^j = W_M^{3j}; W_M^e = W_H^{e / 3} W_M^{e % 3}
     __shared__ double red[T / 64];
     constexpr int M = 3 * H;
     const int tid = threadIdx.x;
@@ -285,26 +286,54 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
     // W_3^{2 n2} for n2 = 0, 1, 2 (indices (2 n2 H) mod M)
     const TC w3_1 = tw[2 * H], w3_2 = tw[H];
     const TC zero{RT(0), RT(0)};
+    // every twiddle of the row comes from LDS: W_H^j for the H-point lines and
+    // W_M^e = W_H^{e / 3} W_M^{e mod 3} for the residue pre- and post-twiddles (one extra
+    // complex multiply), so no global load inside the transforms waits behind the
+    // prefetched inputs (vector memory loads complete in order)
+    const TC wm1 = tw[1], wm2 = tw[2];
+    for (int j = tid; j < H; j += T) twl[j] = tw[3 * j];
+    __syncthreads();
+    auto wM = [&](int e) {  // component-wise selects (struct selects go through scratch)
+        const int q3 = e / 3, r3 = e - 3 * q3;
+        const TC t = twl[q3];
+        const TC m = mul(t, TC{r3 == 1 ? wm1.x : wm2.x, r3 == 1 ? wm1.y : wm2.y});
+        return TC{r3 == 0 ? t.x : m.x, r3 == 0 ? t.y : m.y};
+    };
 
     TC vel[3][PO];     // u + i w at x = n2 + 3 x1
     RT P[3][PO];       // u d_x omega + w d_z omega
     double cmax = 0.0;
 
-#pragma unroll 1
-    for (int ph = 0; ph < ((MODE == 0) ? 2 : 1); ++ph) {
-        const TC *fa = row + (size_t)(2 * ph) * vstride;
+    constexpr int NPH = (MODE == 0) ? 2 : 1;
+    // raw inputs of the two field pairs: the second pair's loads are issued before the
+    // first pair's transforms (whose twiddles come from LDS), so they are in flight
+    // while those run.  Lane l (< 32) of a wave takes mode k = q, lane l ^ 32 mode H - q.
+    TC ra0[PJ], rb0[PJ], ra1[PJ], rb1[PJ];
+    auto issue = [&](const TC *fa, TC *a, TC *b) {
         const TC *fb = fa + vstride;
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const int q = (tid >> 6) * 32 + (lane & 31) + j * (T / 2);
+            const int k = lane < 32 ? q : (q == 0 ? H / 2 : H - q);
+            TC va = zero, vb = zero;  // a branch, not a select (a select of the two sources
+            if (q < H / 2) {          // becomes a flat load from a scratch copy of zero)
+                va = fa[(size_t)k * M];
+                vb = fb[(size_t)k * M];
+            }
+            a[j] = va;
+            b[j] = vb;
+        }
+    };
+    issue(row, ra0, rb0);
+#pragma unroll
+    for (int ph = 0; ph < NPH; ++ph) {
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
             const int q = (tid >> 6) * 32 + (lane & 31) + j * (T / 2);
             const bool valid = q < H / 2;
             const bool hi = lane >= 32;
             const int k = !hi ? q : (q == 0 ? H / 2 : H - q);
-            TC av = zero, bv = zero;
-            if (valid) {
-                av = fa[(size_t)k * M];
-                bv = fb[(size_t)k * M];
-            }
+            const TC av = ph == 0 ? ra0[j] : ra1[j], bv = ph == 0 ? rb0[j] : rb1[j];
             // partner lane (l ^ 32) holds H - k, except in the q = 0 pair (0 and H/2)
             TC am{__shfl_xor(av.x, 32, 64), __shfl_xor(av.y, 32, 64)};
             TC bm{__shfl_xor(bv.x, 32, 64), __shfl_xor(bv.y, 32, 64)};
@@ -322,12 +351,15 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
                     zhi = TC{am.x + bm.y, bm.x - am.y};
                 }
                 lds[pidx(k)] = add(zlo, zhi);
-                lds[LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_1)), tw[opaque(k)]);
-                lds[2 * LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_2)), tw[opaque(2 * k)]);
+                lds[LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_1)), wM(k));
+                lds[2 * LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_2)), wM(2 * k));
             }
         }
+        if constexpr (NPH == 2) {
+            if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
+        }
         __syncthreads();
-        fft_lines<H, LS, 3, T, true>(lds, tw, 3);
+        fft_lines<H, LS, 3, T, true>(lds, twl, 1);
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
@@ -362,7 +394,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             }
         }
         __syncthreads();
-        fft_lines<H, LS, 2, T, false>(lds, tw, 3);
+        fft_lines<H, LS, 2, T, false>(lds, twl, 1);
         TC *g = G + (size_t)z * H;
         const RT half = RT(0.5);
         for (int k = tid; k < H; k += T) {
@@ -372,7 +404,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             const TC F0{half * s.x, half * s.y};
             const TC F1{half * d.y, -half * d.x};  // d / (2 i)
             const TC F2 = B[pidx(k)];
-            g[k] = add(F0, add(mul(F1, tw[opaque(k)]), mul(F2, tw[opaque(2 * k)])));
+            g[k] = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
         }
     }
 
