@@ -103,12 +103,16 @@ __device__ __forceinline__ double inv_g_of(int iz, double gz) { return (iz >= 1 
 template <class TC, int H, int C, int T, int MINW>
 __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
                                            const TC *__restrict__ G, TC *__restrict__ V,
-                                           const TC *__restrict__ twT, const cd *__restrict__ tw, ColArgs a,
+                                           const TC *__restrict__ twT, const cd *__restrict__ tw, const ColArgs a,
                                            BatchArgs bt)
 {
     constexpr int N = 2 * H;
     constexpr int M = 3 * H;
     constexpr int PJ = C * H / T;
+    // CN coefficients: the member's own under batching.  `a` itself is never written, so
+    // a.fields[fi] (a run-time index) is read from the kernel arguments, not from a
+    // private copy of the struct in scratch
+    double VT = a.VT, c2 = a.c2, c3 = a.c3;
     {
         const size_t b = blockIdx.y;
         om += b * bt.s_om;
@@ -116,17 +120,51 @@ __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__rest
         if (G) G += b * bt.s_g;
         V += b * bt.s_v;
         if (bt.coef) {
-            a.VT = bt.coef[3 * b];
-            a.c2 = bt.coef[3 * b + 1];
-            a.c3 = bt.coef[3 * b + 2];
+            VT = bt.coef[3 * b];
+            c2 = bt.coef[3 * b + 1];
+            c3 = bt.coef[3 * b + 2];
         }
     }
     constexpr int LS = padded_len(H);
     static_assert((C * H) % T == 0, "tile must divide evenly over the threads");
     __shared__ TC lds[3 * C * LS];
+    // Small H (the ensemble sizes): every twiddle from LDS -- W_H^j in the transform type for
+    // the H-point lines, and in FP64 for the residue factors W_M^e = W_H^{e / 3} W_M^{e mod 3}
+    // -- so the om, fn loads of the CN update can be issued before the forward transforms
+    // without any load inside them waiting behind those (vector loads complete in order).
+    constexpr bool PF = H <= 512;
+    constexpr bool F64 = std::is_same<TC, cd>::value;
+    __shared__ TC twl[PF ? H : 1];
+    __shared__ cd twd[PF && !F64 ? H : 1];
 
     const int tid = threadIdx.x;
     const int kx0 = xcd_tile(blockIdx.x, gridDim.x) * C;
+    const cd wm1 = tw[1], wm2 = tw[2];
+    if constexpr (PF) {
+        for (int j = tid; j < H; j += T) {
+            twl[j] = twT[3 * j];
+            if constexpr (!F64) twd[j] = tw[3 * j];
+        }
+        __syncthreads();
+    }
+    const cd *tabd = nullptr;
+    if constexpr (PF) tabd = F64 ? reinterpret_cast<const cd *>(twl) : twd;
+    auto wM = [&](int e) -> cd {  // W_M^e, 0 <= e < 2H
+        if constexpr (PF) {
+            const int q3 = e / 3, r3 = e - 3 * q3;
+            const cd t = tabd[q3];
+            const cd m = mul(t, cd{r3 == 1 ? wm1.x : wm2.x, r3 == 1 ? wm1.y : wm2.y});
+            return cd{r3 == 0 ? t.x : m.x, r3 == 0 ? t.y : m.y};
+        } else {
+            return tw[opaque(e)];
+        }
+    };
+    auto lines = [&](auto inv) {  // the 3C H-point lines in LDS
+        if constexpr (PF)
+            fft_lines<H, LS, 3 * C, T, decltype(inv)::value>(lds, twl, 1);
+        else
+            fft_lines<H, LS, 3 * C, T, decltype(inv)::value>(lds, twT, 3);
+    };
 
     // registers: the (lo, hi) pair of each owned (column c, k1): iz = k1 and iz = k1 + H
     cd xlo[PJ], xhi[PJ];
@@ -137,8 +175,23 @@ __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__rest
             const int z1 = z / 3, z2 = z - 3 * z1;
             lds[(z2 * C + c) * LS + pidx(z1)] = G[(size_t)z * H + kx0 + c];
         }
+        // om, fn of the owned modes (lo: iz = k1, hi: iz = k1 + H), in flight during the
+        // forward transforms when the twiddles come from LDS
+        cd olo[PJ], ohi[PJ], flo[PJ], fhi[PJ];
+        if constexpr (PF) {
+#pragma unroll
+            for (int j = 0; j < PJ; ++j) {
+                const int p = tid + j * T;
+                const int c = p / H, k1 = p - (p / H) * H;
+                const size_t io = (size_t)(kx0 + c) * N + k1;
+                olo[j] = om[io];
+                ohi[j] = om[io + H];
+                flo[j] = fn[io];
+                fhi[j] = fn[io + H];
+            }
+        }
         __syncthreads();
-        fft_lines<H, LS, 3 * C, T, false>(lds, twT, 3);
+        lines(std::false_type{});
         // radix-3 combine in FP64 (the FFT outputs are widened)
         const cd w3a = tw[2 * H];  // W_3^2
         const cd w3b = tw[H];      // W_3^4 = W_3
@@ -147,8 +200,8 @@ __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__rest
             const int p = tid + j * T;
             const int c = p / H, k1 = p - (p / H) * H;
             const cd F0 = to_d(lds[(0 * C + c) * LS + pidx(k1)]);
-            const cd F1 = mul(to_d(lds[(1 * C + c) * LS + pidx(k1)]), tw[opaque(k1)]);
-            const cd F2 = mul(to_d(lds[(2 * C + c) * LS + pidx(k1)]), tw[opaque(2 * k1)]);
+            const cd F1 = mul(to_d(lds[(1 * C + c) * LS + pidx(k1)]), wM(k1));
+            const cd F2 = mul(to_d(lds[(2 * C + c) * LS + pidx(k1)]), wM(2 * k1));
             xlo[j] = add(F0, add(F1, F2));
             xhi[j] = add(F0, add(mul(F1, w3a), mul(F2, w3b)));
         }
@@ -169,11 +222,18 @@ __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__rest
                 const size_t io = (size_t)kx * N + iz;
                 if (a.fext) FN = add(FN, a.fext[io]);
                 if (iz == H) FN = mk(0.0, 0.0);
-                const cd o = om[io], f = fn[io];
-                const double ARG = a.VT * K2;
+                cd o, f;
+                if constexpr (PF) {
+                    o = half ? ohi[j] : olo[j];
+                    f = half ? fhi[j] : flo[j];
+                } else {
+                    o = om[io];
+                    f = fn[io];
+                }
+                const double ARG = VT * K2;
                 const double DEN = 1.0 + ARG;
                 const double c1 = 1.0 - ARG;
-                const cd num = mk(c1 * o.x + a.c2 * FN.x + a.c3 * f.x, c1 * o.y + a.c2 * FN.y + a.c3 * f.y);
+                const cd num = mk(c1 * o.x + c2 * FN.x + c3 * f.x, c1 * o.y + c2 * FN.y + c3 * f.y);
                 const cd on = mk(num.x / DEN, num.y / DEN);
                 om[io] = on;
                 fn[io] = FN;
@@ -227,11 +287,11 @@ __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__rest
             const cd shi = field_spec(f, xhi[j], kx, k1 + H, ax, ghi, ik_hi[j], ighi);
             // residue z2: W_M^{-z2 k1} (S[k1] + S[k1 + 2H] W_3^{-2 z2})
             lds[(0 * C + c) * LS + pidx(k1)] = from_d<TC>(add(slo, shi));
-            lds[(1 * C + c) * LS + pidx(k1)] = from_d<TC>(mulc(add(slo, mulc(shi, tw[2 * H])), tw[opaque(k1)]));
-            lds[(2 * C + c) * LS + pidx(k1)] = from_d<TC>(mulc(add(slo, mulc(shi, tw[H])), tw[opaque(2 * k1)]));
+            lds[(1 * C + c) * LS + pidx(k1)] = from_d<TC>(mulc(add(slo, mulc(shi, tw[2 * H])), wM(k1)));
+            lds[(2 * C + c) * LS + pidx(k1)] = from_d<TC>(mulc(add(slo, mulc(shi, tw[H])), wM(2 * k1)));
         }
         __syncthreads();
-        fft_lines<H, LS, 3 * C, T, true>(lds, twT, 3);
+        lines(std::true_type{});
         TC *out = V + (size_t)fi * a.vstride + (size_t)kx0 * M;
         for (int e = tid; e < C * M; e += T) {
             const int c = e / M, zz = e - c * M;
