@@ -485,6 +485,135 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
 }
 
 // =====================================================================================
+// Row kernel for small H (128 .. 512): R consecutive rows per workgroup, MODE 0 only (the
+// step).  V is column-major, so the R rows' values of one mode kx sit side by side (16 R
+// bytes in FP64); lane l of a wave takes row z0 + (l mod 32) mod R and mode slot
+// (l mod 32) / R, so each load instruction reads R rows x 32 / R modes in runs of R values
+// instead of one value per line.  Per row the work is k_row's, line for line (3R lines per
+// transform, the same engine and twiddles), so the results are bitwise those of k_row.
+// =====================================================================================
+template <class TC, int H, int T, int R, int MINW>
+__global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, size_t vstride, TC *__restrict__ G,
+                                              const TC *__restrict__ tw, BatchArgs bt)
+{
+    {
+        const size_t b = blockIdx.y;
+        V += b * bt.s_v;
+        G += b * bt.s_g;
+    }
+    using RT = decltype(TC::x);
+    constexpr int LS = padded_len(H);
+    constexpr int QW = T / (2 * R);   // modes q (pairs q, H - q) per lane sweep of the workgroup
+    constexpr int PJ = (H / 2) / QW;  // pairs per lane
+    constexpr int PO = R * H / T;     // outputs per thread and residue
+    static_assert(T % 64 == 0 && 32 % R == 0 && (H / 2) % QW == 0 && (R * H) % T == 0, "row-group tiling");
+    __shared__ TC lds[3 * R * LS];
+    __shared__ TC twl[H];
+    constexpr int M = 3 * H;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int z0 = xcd_tile(blockIdx.x, gridDim.x) * R;
+    const int r = (lane & 31) % R, sq = (lane & 31) / R;
+    const TC *row = V + z0 + r;  // element (f, k) of row z0 + r at f * vstride + k * M
+    const TC w3_1 = tw[2 * H], w3_2 = tw[H];
+    const TC zero{RT(0), RT(0)};
+    const TC wm1 = tw[1], wm2 = tw[2];
+    for (int j = tid; j < H; j += T) twl[j] = tw[3 * j];
+    __syncthreads();
+    auto wM = [&](int e) {  // W_M^e from the table, as in k_row
+        const int q3 = e / 3, r3 = e - 3 * q3;
+        const TC t = twl[q3];
+        const TC m = mul(t, TC{r3 == 1 ? wm1.x : wm2.x, r3 == 1 ? wm1.y : wm2.y});
+        return TC{r3 == 0 ? t.x : m.x, r3 == 0 ? t.y : m.y};
+    };
+
+    TC vel[3][PO];
+    RT P[3][PO];
+    TC ra0[PJ], rb0[PJ], ra1[PJ], rb1[PJ];
+    auto mode_of = [&](int j, int *q) {
+        *q = (tid >> 6) * (32 / R) + sq + j * QW;
+        return lane < 32 ? *q : (*q == 0 ? H / 2 : H - *q);
+    };
+    auto issue = [&](const TC *fa, TC *a, TC *b) {
+        const TC *fb = fa + vstride;
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            int q;
+            const int k = mode_of(j, &q);
+            a[j] = fa[(size_t)k * M];
+            b[j] = fb[(size_t)k * M];
+        }
+    };
+    issue(row, ra0, rb0);
+#pragma unroll
+    for (int ph = 0; ph < 2; ++ph) {
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            int q;
+            const int k = mode_of(j, &q);
+            const TC av = ph == 0 ? ra0[j] : ra1[j], bv = ph == 0 ? rb0[j] : rb1[j];
+            TC am{__shfl_xor(av.x, 32, 64), __shfl_xor(av.y, 32, 64)};
+            TC bm{__shfl_xor(bv.x, 32, 64), __shfl_xor(bv.y, 32, 64)};
+            if (k == H / 2) {
+                am = av;
+                bm = bv;
+            }
+            TC zlo, zhi;
+            if (k == 0) {
+                zlo = TC{av.x, bv.x};
+                zhi = zero;
+            } else {
+                zlo = TC{av.x - bv.y, av.y + bv.x};
+                zhi = TC{am.x + bm.y, bm.x - am.y};
+            }
+            lds[(0 * R + r) * LS + pidx(k)] = add(zlo, zhi);
+            lds[(1 * R + r) * LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_1)), wM(k));
+            lds[(2 * R + r) * LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_2)), wM(2 * k));
+        }
+        if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
+        __syncthreads();
+        fft_lines<H, LS, 3 * R, T, true>(lds, twl, 1);
+#pragma unroll
+        for (int j = 0; j < PO; ++j) {
+            const int e = tid + j * T;
+            const int rr = e / H, x1 = e - rr * H;
+#pragma unroll
+            for (int n2 = 0; n2 < 3; ++n2) {
+                const TC v = lds[(n2 * R + rr) * LS + pidx(x1)];
+                if (ph == 0)
+                    vel[n2][j] = v;
+                else
+                    P[n2][j] = vel[n2][j].x * v.x + vel[n2][j].y * v.y;
+            }
+        }
+        __syncthreads();
+    }
+    // x-forward of each row's real product: lines A_r = P0 + i P1 (line r) and B_r = P2
+    // (line R + r), then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
+#pragma unroll
+    for (int j = 0; j < PO; ++j) {
+        const int e = tid + j * T;
+        const int rr = e / H, x1 = e - rr * H;
+        lds[rr * LS + pidx(x1)] = TC{P[0][j], P[1][j]};
+        lds[(R + rr) * LS + pidx(x1)] = TC{P[2][j], RT(0)};
+    }
+    __syncthreads();
+    fft_lines<H, LS, 2 * R, T, false>(lds, twl, 1);
+    const RT half = RT(0.5);
+    for (int e = tid; e < R * H; e += T) {
+        const int rr = e / H, k = e - rr * H;
+        const TC *A = lds + rr * LS;
+        const TC za = A[pidx(k)];
+        const TC zm = conj(A[pidx(k == 0 ? 0 : H - k)]);
+        const TC s = add(za, zm), d = sub(za, zm);
+        const TC F0{half * s.x, half * s.y};
+        const TC F1{half * d.y, -half * d.x};  // d / (2 i)
+        const TC F2 = lds[(R + rr) * LS + pidx(k)];
+        G[(size_t)(z0 + rr) * H + k] = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
+    }
+}
+
+// =====================================================================================
 // Physical plane of one or two z-inverted fields (diagnostics / output dumps).
 // out_mode 0: out0 = Re, out1 = Im (if non-null); out_mode 1: out0 = |a + i b|.
 // =====================================================================================
@@ -731,6 +860,15 @@ constexpr bool has_variants(int H) { return H >= 16 && H <= 512 && (H & (H - 1))
 template <class TC, int H, class Q>
 hipError_t launch_cfg(const LaunchReq &r);
 
+// rows per workgroup and threads of k_row_mr, timed on the ensemble (192 x N=256 +
+// 64 x N=512): two rows per workgroup beat four and eight (fewer VGPRs and LDS, more
+// workgroups per CU outweigh the longer runs of V values)
+template <int H>
+struct MrCfg {
+    static constexpr int R = 2;
+    static constexpr int T = H == 128 ? 128 : 256;
+};
+
 // FP32 transforms at H = 2048 (N = 4096): the lines take half the LDS and the kernels fewer
 // VGPRs, so both run 1024 threads per workgroup (timed: row 0.74 -> 0.70 ms, column
 // 0.50 -> 0.47 ms; results bitwise unchanged -- the same arithmetic per element)
@@ -771,9 +909,19 @@ hipError_t launch_cfg(const LaunchReq &r)
                                r.fn, static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col, r.batch);
             break;
         case LAUNCH_ROW:
-            hipLaunchKernelGGL((k_row<TC, H, Q::TR, 0, WR>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
-                               static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
-                               r.DX, r.DZ, r.batch);
+            // H = 128 .. 512 at the default row threads: row groups (k_row_mr); the variants
+            // with doubled row threads keep one row per workgroup, so the ensemble's
+            // autotuner can choose between the two
+            if constexpr (H >= 128 && H <= 512 && (H & (H - 1)) == 0 && Q::TR == Cfg<H>::TR) {
+                if (r.cflrow) return hipErrorInvalidValue;  // the step's row pass computes no CFL
+                constexpr int RR = MrCfg<H>::R, TT = MrCfg<H>::T;
+                hipLaunchKernelGGL((k_row_mr<TC, H, TT, RR, 1>), dim3(M / RR, nb), dim3(TT), 0, r.stream,
+                                   static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.batch);
+            } else {
+                hipLaunchKernelGGL((k_row<TC, H, Q::TR, 0, WR>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
+                                   static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
+                                   r.DX, r.DZ, r.batch);
+            }
             break;
         case LAUNCH_ROW_CFL:
             hipLaunchKernelGGL((k_row<TC, H, Q::TR, 1, WR>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
