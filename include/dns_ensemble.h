@@ -57,7 +57,9 @@ typedef struct {
     int                   use_graph;     /* 1: capture a block of pair_every steps in a HIP graph */
     int                   device;
     int                   autotune;      /* 1: dns_ens_init times the built launch configurations
-                                            of each size and keeps the fastest               */
+                                            of each size and keeps the fastest; 2 + v: every
+                                            size runs configuration v (0 <= v < 4 where the size
+                                            has four, else 0), untimed; 0: configuration 0    */
 } dns_ens_config;
 
 /* Allocates every member's state and work arrays on the device; no IC yet. */

@@ -354,10 +354,14 @@ class Solver:
 class Ensemble:
     """Many independent runs advanced together (include/dns_ensemble.h): members given as
     (N, Re, K0, cflnum, seed) tuples; training pairs every ``pair_every`` steps on the
-    N / ``filter_ratio`` grid, channels (filtered omega, u, w, subgrid term)."""
+    N / ``filter_ratio`` grid, channels (filtered omega, u, w, subgrid term).
+
+    ``autotune``: True times the launch configurations of each size and keeps the fastest;
+    ``configuration=v`` runs configuration v (0..3) on every size without timing."""
 
     def __init__(self, members, pair_every: int = 20, filter_ratio: int = 8, max_pairs: int = 100,
-                 use_graph: bool = True, device: int = 0, autotune: bool = False):
+                 use_graph: bool = True, device: int = 0, autotune: bool = False,
+                 configuration: int | None = None):
         L = load_library()
         self.members = [tuple(m) for m in members]
         arr = (EnsMember * len(self.members))()
@@ -365,7 +369,8 @@ class Ensemble:
             arr[i] = EnsMember(int(N), float(Re), float(K0), float(cfl), int(seed))
         self._arr = arr
         cfg = EnsConfig(len(self.members), arr, int(pair_every), int(filter_ratio), int(max_pairs),
-                        1 if use_graph else 0, int(device), 1 if autotune else 0)
+                        1 if use_graph else 0, int(device),
+                        2 + int(configuration) if configuration is not None else (1 if autotune else 0))
         h = _P()
         st = L.dns_ens_create(ctypes.byref(cfg), ctypes.byref(h))
         if st:

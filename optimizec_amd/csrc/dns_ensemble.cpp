@@ -342,6 +342,8 @@ extern "C" dns_status dns_ens_create(const dns_ens_config *cfg, dns_ens **out)
     if (cfg->pair_every < 1) return efail(nullptr, DNS_E_ARG, "pair_every must be >= 1");
     if (cfg->filter_ratio < 2) return efail(nullptr, DNS_E_ARG, "filter_ratio must be >= 2");
     if (cfg->max_pairs < 0) return efail(nullptr, DNS_E_ARG, "max_pairs must be >= 0");
+    if (cfg->autotune < 0 || cfg->autotune > 5)
+        return efail(nullptr, DNS_E_ARG, "autotune %d: expected 0, 1 or 2 + configuration (0..3)", cfg->autotune);
     for (int m = 0; m < cfg->nmembers; ++m) {
         const dns_ens_member &mb = cfg->members[m];
         const int N = mb.N;
@@ -365,7 +367,7 @@ extern "C" dns_status dns_ens_create(const dns_ens_config *cfg, dns_ens **out)
     e->ratio = cfg->filter_ratio;
     e->max_pairs = cfg->max_pairs;
     e->use_graph = cfg->use_graph ? 1 : 0;
-    e->autotune = cfg->autotune ? 1 : 0;
+    e->autotune = cfg->autotune;
     e->members.assign(cfg->members, cfg->members + cfg->nmembers);
     auto bail = [&](dns_status st) {
         g_ens_create_error = e->err;
@@ -477,7 +479,14 @@ extern "C" dns_status dns_ens_init(dns_ens *e)
         }
         EHIP(hipMemcpy(g.sc, sc.data(), sizeof(EnsScalars) * g.count, hipMemcpyHostToDevice));
         ESTEP(refresh_group(e, g));
-        if (e->autotune) ESTEP(autotune_group(e, g));
+        if (e->autotune == 1) ESTEP(autotune_group(e, g));
+        if (e->autotune >= 2) {
+            const int v = e->autotune - 2;
+            if (v >= dns_kernels_variants(g.H, 0))
+                return efail(e, DNS_E_ARG, "N=%d has %d launch configuration(s), %d requested", g.N,
+                             dns_kernels_variants(g.H, 0), v);
+            g.variant = v;
+        }
         ESTEP(issue_nextdt(e, g));
     }
     EHIP(hipMemsetAsync(e->slot_d, 0, sizeof(int), e->stream));

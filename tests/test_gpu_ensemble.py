@@ -62,6 +62,32 @@ def test_members_bitwise_equal_single_runs(use_graph, autotune):
     e.close()
 
 
+@pytest.mark.parametrize("configuration", [1, 2, 3])
+def test_forced_configurations_bitwise_equal_single_runs(configuration):
+    """Configurations 2 and 3 double the row threads and keep one row per workgroup
+    (k_row); single runs at H = 128 .. 512 use the two-row kernel (k_row_mr).  Both do the
+    same arithmetic per element, so the states agree bit for bit."""
+    members = [(256, 2e4, 8.0, 0.1, 11), (512, 5e4, 4.0, 0.1, 12)]
+    print(f"ensemble configuration {configuration} vs single runs, N=256 and 512", flush=True)
+    e = oa.Ensemble(members, pair_every=EVERY, filter_ratio=RATIO, max_pairs=2, configuration=configuration)
+    e.init()
+    assert [e.tuning(m)[0] for m in range(len(members))] == [configuration] * len(members)
+    e.advance(30)
+    e.synchronize()
+    for m, mem in enumerate(members):
+        s, _ = _standalone(mem, 30)
+        assert e.scalars(m).dt == s.scalars().dt, m
+        assert np.array_equal(e.om2(m), s.om2()), m
+        assert np.array_equal(e.fnm1(m), s.fnm1()), m
+        s.close()
+    e.close()
+
+
+def test_configuration_out_of_range_rejected():
+    with pytest.raises(oa.DnsError):
+        oa.Ensemble([(256, 1e3, 4.0, 0.1, 1)], configuration=4)
+
+
 def _phys(S, n):
     full = np.zeros((n, n // 2 + 1), dtype=np.complex128)
     full[:, : n // 2] = S
