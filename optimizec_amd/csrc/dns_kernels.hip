@@ -361,7 +361,6 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
     };
 
     TC vel[3][PO];     // u + i w at x = n2 + 3 x1
-    RT P[3][PO];       // u d_x omega + w d_z omega
     double cmax = 0.0;
 
     constexpr int NPH = (MODE == 0) ? 2 : 1;
@@ -424,16 +423,26 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
         for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
             if (x1 < H) {
+                if (ph == 0) {
 #pragma unroll
-                for (int n2 = 0; n2 < 3; ++n2) {
-                    const TC v = lds[n2 * LS + pidx(x1)];
-                    if (ph == 0) {
+                    for (int n2 = 0; n2 < 3; ++n2) {
+                        const TC v = lds[n2 * LS + pidx(x1)];
                         vel[n2][j] = v;
                         const double cfl = fabs(double(v.x)) / DX + fabs(double(v.y)) / DZ;
                         cmax = cfl > cmax ? cfl : cmax;
-                    } else {
-                        P[n2][j] = vel[n2][j].x * v.x + vel[n2][j].y * v.y;
                     }
+                } else {
+                    // the real product P = u d_x omega + w d_z omega, written straight into
+                    // the x-forward's lines A = P0 + i P1 (line 0) and B = P2 (line 1): this
+                    // thread alone reads and writes position x1 of the lines
+                    RT P[3];
+#pragma unroll
+                    for (int n2 = 0; n2 < 3; ++n2) {
+                        const TC v = lds[n2 * LS + pidx(x1)];
+                        P[n2] = vel[n2][j].x * v.x + vel[n2][j].y * v.y;
+                    }
+                    lds[pidx(x1)] = TC{P[0], P[1]};
+                    lds[LS + pidx(x1)] = TC{P[2], RT(0)};
                 }
             }
         }
@@ -445,15 +454,6 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
         // A = P0 + i P1, B = P2), then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
         TC *A = lds;
         TC *B = lds + LS;
-#pragma unroll
-        for (int j = 0; j < PO; ++j) {
-            const int x1 = tid + j * T;
-            if (x1 < H) {
-                A[pidx(x1)] = TC{P[0][j], P[1][j]};
-                B[pidx(x1)] = TC{P[2][j], RT(0)};
-            }
-        }
-        __syncthreads();
         fft_lines<H, LS, 2, T, false>(lds, twl, 1);
         TC *g = G + (size_t)z * H;
         const RT half = RT(0.5);
@@ -485,8 +485,8 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
 }
 
 // =====================================================================================
-// Row kernel for small H (128 .. 512): R consecutive rows per workgroup, MODE 0 only (the
-// step).  V is column-major, so the R rows' values of one mode kx sit side by side (16 R
+// Row kernel over row groups (H = 128 .. 512, and FP32 at H = 2048): R consecutive rows
+// per workgroup, MODE 0 only (the step).  V is column-major, so the R rows' values of one mode kx sit side by side (16 R
 // bytes in FP64); lane l of a wave takes row z0 + (l mod 32) mod R and mode slot
 // (l mod 32) / R, so each load instruction reads R rows x 32 / R modes in runs of R values
 // instead of one value per line.  Per row the work is k_row's, line for line (3R lines per
@@ -528,7 +528,6 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
     };
 
     TC vel[3][PO];
-    RT P[3][PO];
     TC ra0[PJ], rb0[PJ], ra1[PJ], rb1[PJ];
     auto mode_of = [&](int j, int *q) {
         *q = (tid >> 6) * (32 / R) + sq + j * QW;
@@ -544,6 +543,11 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
             b[j] = fb[(size_t)k * M];
         }
     };
+    // LDS line of residue n2 of row r: residue-major (all rows in one transform call) or,
+    // at H = 2048 (FP32), row-major with one transform call per row, which halves the
+    // butterflies a thread holds across a stage's barrier (the kernel then fits 128 VGPRs)
+    constexpr bool BYROW = H == 2048;
+    auto line = [](int r, int n2) { return BYROW ? 3 * r + n2 : n2 * R + r; };
     issue(row, ra0, rb0);
 #pragma unroll
     for (int ph = 0; ph < 2; ++ph) {
@@ -566,49 +570,58 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
                 zlo = TC{av.x - bv.y, av.y + bv.x};
                 zhi = TC{am.x + bm.y, bm.x - am.y};
             }
-            lds[(0 * R + r) * LS + pidx(k)] = add(zlo, zhi);
-            lds[(1 * R + r) * LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_1)), wM(k));
-            lds[(2 * R + r) * LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_2)), wM(2 * k));
+            lds[line(r, 0) * LS + pidx(k)] = add(zlo, zhi);
+            lds[line(r, 1) * LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_1)), wM(k));
+            lds[line(r, 2) * LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_2)), wM(2 * k));
         }
         if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
         __syncthreads();
-        fft_lines<H, LS, 3 * R, T, true>(lds, twl, 1);
+        if constexpr (BYROW) {
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) fft_lines<H, LS, 3, T, true>(lds + 3 * rr * LS, twl, 1);
+        } else {
+            fft_lines<H, LS, 3 * R, T, true>(lds, twl, 1);
+        }
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int e = tid + j * T;
             const int rr = e / H, x1 = e - rr * H;
+            if (ph == 0) {
 #pragma unroll
-            for (int n2 = 0; n2 < 3; ++n2) {
-                const TC v = lds[(n2 * R + rr) * LS + pidx(x1)];
-                if (ph == 0)
-                    vel[n2][j] = v;
-                else
-                    P[n2][j] = vel[n2][j].x * v.x + vel[n2][j].y * v.y;
+                for (int n2 = 0; n2 < 3; ++n2) vel[n2][j] = lds[line(rr, n2) * LS + pidx(x1)];
+            } else {
+                // the real product u d_x omega + w d_z omega, written straight into the
+                // x-forward's lines A_r = P0 + i P1 (line (r, 0)) and B_r = P2 (line (r, 1)):
+                // this thread alone reads and writes position x1 of row rr's lines
+                RT P[3];
+#pragma unroll
+                for (int n2 = 0; n2 < 3; ++n2) {
+                    const TC v = lds[line(rr, n2) * LS + pidx(x1)];
+                    P[n2] = vel[n2][j].x * v.x + vel[n2][j].y * v.y;
+                }
+                lds[line(rr, 0) * LS + pidx(x1)] = TC{P[0], P[1]};
+                lds[line(rr, 1) * LS + pidx(x1)] = TC{P[2], RT(0)};
             }
         }
         __syncthreads();
     }
-    // x-forward of each row's real product: lines A_r = P0 + i P1 (line r) and B_r = P2
-    // (line R + r), then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
+    // x-forward of each row's product, then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
+    if constexpr (BYROW) {
 #pragma unroll
-    for (int j = 0; j < PO; ++j) {
-        const int e = tid + j * T;
-        const int rr = e / H, x1 = e - rr * H;
-        lds[rr * LS + pidx(x1)] = TC{P[0][j], P[1][j]};
-        lds[(R + rr) * LS + pidx(x1)] = TC{P[2][j], RT(0)};
+        for (int rr = 0; rr < R; ++rr) fft_lines<H, LS, 2, T, false>(lds + 3 * rr * LS, twl, 1);
+    } else {
+        fft_lines<H, LS, 2 * R, T, false>(lds, twl, 1);  // lines (r, 0), (r, 1) are 0 .. 2R - 1
     }
-    __syncthreads();
-    fft_lines<H, LS, 2 * R, T, false>(lds, twl, 1);
     const RT half = RT(0.5);
     for (int e = tid; e < R * H; e += T) {
         const int rr = e / H, k = e - rr * H;
-        const TC *A = lds + rr * LS;
+        const TC *A = lds + line(rr, 0) * LS;
         const TC za = A[pidx(k)];
         const TC zm = conj(A[pidx(k == 0 ? 0 : H - k)]);
         const TC s = add(za, zm), d = sub(za, zm);
         const TC F0{half * s.x, half * s.y};
         const TC F1{half * d.y, -half * d.x};  // d / (2 i)
-        const TC F2 = lds[(R + rr) * LS + pidx(k)];
+        const TC F2 = lds[line(rr, 1) * LS + pidx(k)];
         G[(size_t)(z0 + rr) * H + k] = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
     }
 }
@@ -862,12 +875,22 @@ hipError_t launch_cfg(const LaunchReq &r);
 
 // rows per workgroup and threads of k_row_mr, timed on the ensemble (192 x N=256 +
 // 64 x N=512): two rows per workgroup beat four and eight (fewer VGPRs and LDS, more
-// workgroups per CU outweigh the longer runs of V values)
-template <int H>
+// workgroups per CU outweigh the longer runs of V values).  FP32 at H = 2048 (N = 4096):
+// two rows, 1024 threads, row pass 0.58 -> 0.50 ms (512 threads: 0.51 ms); two FP64 rows
+// of that length do not fit in LDS.
+template <class TC, int H>
 struct MrCfg {
     static constexpr int R = 2;
-    static constexpr int T = H == 128 ? 128 : 256;
+    static constexpr int T = H == 128 ? 128 : H == 2048 ? 1024 : 256;
 };
+// which row kernel the step's row pass runs: row groups at H = 128 .. 512 (the default row
+// threads) and for FP32 at H = 2048, one row per workgroup otherwise
+template <class TC, int H, class Q>
+constexpr bool use_row_groups()
+{
+    if constexpr (H == 2048) return std::is_same<TC, cf>::value;
+    else return H >= 128 && H <= 512 && (H & (H - 1)) == 0 && Q::TR == Cfg<H>::TR;
+}
 
 // FP32 transforms at H = 2048 (N = 4096): the lines take half the LDS and the kernels fewer
 // VGPRs, so both run 1024 threads per workgroup (timed: row 0.74 -> 0.70 ms, column
@@ -912,9 +935,9 @@ hipError_t launch_cfg(const LaunchReq &r)
             // H = 128 .. 512 at the default row threads: row groups (k_row_mr); the variants
             // with doubled row threads keep one row per workgroup, so the ensemble's
             // autotuner can choose between the two
-            if constexpr (H >= 128 && H <= 512 && (H & (H - 1)) == 0 && Q::TR == Cfg<H>::TR) {
+            if constexpr (use_row_groups<TC, H, Q>()) {
                 if (r.cflrow) return hipErrorInvalidValue;  // the step's row pass computes no CFL
-                constexpr int RR = MrCfg<H>::R, TT = MrCfg<H>::T;
+                constexpr int RR = MrCfg<TC, H>::R, TT = MrCfg<TC, H>::T;
                 hipLaunchKernelGGL((k_row_mr<TC, H, TT, RR, 1>), dim3(M / RR, nb), dim3(TT), 0, r.stream,
                                    static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.batch);
             } else {

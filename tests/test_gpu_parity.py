@@ -126,6 +126,26 @@ def test_4096_short_parity_and_invariants(oracle_mod):
     s.close()
 
 
+def test_4096_fp32_transforms_against_fp64():
+    """N=4096 with FP32 transforms (its row pass runs two rows per workgroup) against the
+    FP64 path, which the test above pins to the oracle: relative L2 vorticity error after
+    100 steps <= 1e-5, and above 1e-12 (the transforms really ran in FP32)."""
+    N, steps, update = 4096, 100, 20
+    runs = []
+    for prec in (oa.PREC_FP64, oa.PREC_FP32):
+        print(f"N={N} precision={prec}: {steps} steps", flush=True)
+        s = oa.Solver(N, 1e5, 4.0, 0.1, precision=prec)
+        s.init_ic(oa.IC_K4EXP, 0)
+        s.start()
+        r = s.run(steps, update)
+        runs.append((s.om2(), r))
+        s.close()
+    err = relerr(runs[1][0], runs[0][0])
+    print(f"FP32 transforms N={N}: rel L2 vorticity error vs FP64 after {steps} steps {err:.2e}")
+    assert 1e-12 < err < 1e-5
+    assert np.max(np.abs(runs[1][1][:, 5:] - runs[0][1][:, 5:]) / np.abs(runs[0][1][:, 5:])) < 1e-6
+
+
 def test_single_shell_decay_gpu():
     """All modes on |k| = 5: the nonlinear term vanishes and each step multiplies omega
     by (1 - a)/(1 + a) with a = nu dt K^2 / 2 (closed form, any N)."""
