@@ -83,32 +83,49 @@ __host__ __device__ constexpr int pick_radix57(int m)
 {
     return (m % 15 == 0) ? 15 : (m % 14 == 0) ? 14 : (m % 35 == 0) ? 35 : (m % 21 == 0) ? 21 : (m % 10 == 0) ? 10 : 0;
 }
-__host__ __device__ constexpr int pick_radix(int m)
+// Radix plans.  maxr = 8 (the default): radix 8 while 8 divides, then 4, the composites,
+// 2, 3, 5, 7.  maxr = 16: the power-of-two part 2^p (p >= 3) runs in the fewest stages of
+// radix <= 16, ceil(p / 4), with the radices as even as possible, largest first
+// (2048 = 16 x 16 x 8, 1024 = 16 x 8 x 8, 512 = 8 x 8 x 8, 128 = 16 x 8): fewer LDS passes
+// and barriers for more registers per butterfly, which pays only in some kernels (chosen
+// by timing where the engine is called).
+__host__ __device__ constexpr int pow2_radix(int m, int maxr)
 {
-    return (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (pick_radix57(m) != 0) ? pick_radix57(m) : (m % 2 == 0) ? 2
-         : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : (m % 7 == 0) ? 7 : m;
+    if (maxr < 16) return 8;
+    int p = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++p;
+    }
+    const int st = (p + 3) / 4;
+    return 1 << ((p + st - 1) / st);
 }
-__host__ __device__ constexpr int stage_radix(int L, int s)
+__host__ __device__ constexpr int pick_radix(int m, int maxr = 8)
+{
+    return (m % 8 == 0) ? pow2_radix(m, maxr) : (m % 4 == 0) ? 4 : (pick_radix57(m) != 0) ? pick_radix57(m)
+         : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : (m % 7 == 0) ? 7 : m;
+}
+__host__ __device__ constexpr int stage_radix(int L, int s, int maxr = 8)
 {
     int m = L;
-    for (int i = 0; i < s; ++i) m /= pick_radix(m);
-    return m == 1 ? 1 : pick_radix(m);
+    for (int i = 0; i < s; ++i) m /= pick_radix(m, maxr);
+    return m == 1 ? 1 : pick_radix(m, maxr);
 }
-__host__ __device__ constexpr int stage_span(int L, int s)  // product of earlier radices
+__host__ __device__ constexpr int stage_span(int L, int s, int maxr = 8)  // product of earlier radices
 {
     int ns = 1, m = L;
     for (int i = 0; i < s; ++i) {
-        const int r = pick_radix(m);
+        const int r = pick_radix(m, maxr);
         ns *= r;
         m /= r;
     }
     return ns;
 }
-__host__ __device__ constexpr int num_stages(int L)
+__host__ __device__ constexpr int num_stages(int L, int maxr = 8)
 {
     int n = 0, m = L;
     while (m > 1) {
-        m /= pick_radix(m);
+        m /= pick_radix(m, maxr);
         ++n;
     }
     return n;
@@ -211,6 +228,37 @@ __device__ __forceinline__ void dft(C *v)
         v[5] = sub(t2, u2);
         v[3] = add(t3, u3);
         v[4] = sub(t3, u3);
+    } else if constexpr (R == 16) {
+        // 4 x 4: DFT-4 over n1 of x[4 n1 + n2] for each n2, times W_16^(n2 k1), then DFT-4
+        // over n2 for each k1 -> X[k1 + 4 k2]
+        constexpr double c1 = 0.92387953251128675612818318939678828682;  // cos(pi / 8)
+        constexpr double s1 = 0.38268343236508977172845998403039886676;  // sin(pi / 8)
+        constexpr double h = 0.70710678118654752440084436210484903928;
+        C y[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            C t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+            dft<4, INV, C>(t);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
+        }
+        // W_16^e = cos(2 pi e / 16) + i S sin(2 pi e / 16) for e = n2 k1 in {1, 2, 3, 4, 6, 9}
+        y[1][1] = mul(y[1][1], C{RT(c1), RT(S * s1)});
+        y[1][2] = mul(y[1][2], C{RT(h), RT(S * h)});
+        y[1][3] = mul(y[1][3], C{RT(s1), RT(S * c1)});
+        y[2][1] = mul(y[2][1], C{RT(h), RT(S * h)});
+        y[2][2] = muli<S>(y[2][2]);
+        y[2][3] = mul(y[2][3], C{RT(-h), RT(S * h)});
+        y[3][1] = mul(y[3][1], C{RT(s1), RT(S * c1)});
+        y[3][2] = mul(y[3][2], C{RT(-h), RT(S * h)});
+        y[3][3] = mul(y[3][3], C{RT(-c1), RT(-S * s1)});
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            C t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+            dft<4, INV, C>(t);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+        }
     } else if constexpr (R == 10) {
         dft_pfa<2, 5, INV, C>(v);
     } else if constexpr (R == 14) {
@@ -322,12 +370,12 @@ __device__ __forceinline__ void stage(C *buf, const C *__restrict__ tw, int twmu
     __syncthreads();
 }
 
-template <int L, int LS, int NL, int T, bool INV, int S, class C>
+template <int L, int LS, int NL, int T, bool INV, int S, int MAXR, class C>
 __device__ __forceinline__ void stages_from(C *buf, const C *__restrict__ tw, int twmul)
 {
-    if constexpr (S < num_stages(L)) {
-        stage<L, LS, NL, T, stage_radix(L, S), stage_span(L, S), INV, C>(buf, tw, twmul);
-        stages_from<L, LS, NL, T, INV, S + 1, C>(buf, tw, twmul);
+    if constexpr (S < num_stages(L, MAXR)) {
+        stage<L, LS, NL, T, stage_radix(L, S, MAXR), stage_span(L, S, MAXR), INV, C>(buf, tw, twmul);
+        stages_from<L, LS, NL, T, INV, S + 1, MAXR, C>(buf, tw, twmul);
     }
 }
 
@@ -335,13 +383,15 @@ __device__ __forceinline__ void stages_from(C *buf, const C *__restrict__ tw, in
 // buf[l * LS + pidx(i)]; LS >= padded_len(L).  Callers barrier before (data
 // written) -- the first stage reads after the caller's __syncthreads -- and the
 // engine ends with a barrier, so results are visible to every thread on return.
-// tw may point to global memory or to an LDS table.
-template <int L, int LS, int NL, int T, bool INV, class C>
+// tw may point to global memory or to an LDS table.  MAXR selects the radix plan (8 or 16,
+// see pick_radix).
+template <int L, int LS, int NL, int T, bool INV, int MAXR = 8, class C>
 __device__ __forceinline__ void fft_lines(C *buf, const C *__restrict__ tw, int twmul)
 {
     static_assert(smooth7(L), "line length must factor into 2, 3, 5, 7");
     static_assert(LS >= padded_len(L), "line stride must hold the padded line");
-    stages_from<L, LS, NL, T, INV, 0, C>(buf, tw, twmul);
+    static_assert(MAXR == 8 || MAXR == 16, "radix plans: 8 or 16");
+    stages_from<L, LS, NL, T, INV, 0, MAXR, C>(buf, tw, twmul);
 }
 
 }  // namespace dnsfft

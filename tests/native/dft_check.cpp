@@ -53,9 +53,19 @@ int main()
     report<10>();
     report<14>();
     report<15>();
+    report<16>();
     report<21>();
     report<35>();
     // the 210-point lines of the 44100-point transform run as 15 x 14
     std::printf("plan210 %d %d %d\n", num_stages(210), stage_radix(210, 0), stage_radix(210, 1));
+    // powers of two: radix 8 first by default; with radices up to 16 the fewest stages,
+    // largest radix first
+    std::printf("plan2048 %d %d %d %d %d\n", num_stages(2048), stage_radix(2048, 0), stage_radix(2048, 1),
+                stage_radix(2048, 2), stage_radix(2048, 3));
+    std::printf("plan2048r16 %d %d %d %d\n", num_stages(2048, 16), stage_radix(2048, 0, 16), stage_radix(2048, 1, 16),
+                stage_radix(2048, 2, 16));
+    std::printf("plan1536r16 %d %d %d %d %d\n", num_stages(1536, 16), stage_radix(1536, 0, 16),
+                stage_radix(1536, 1, 16), stage_radix(1536, 2, 16), stage_radix(1536, 3, 16));
+    std::printf("plan128r16 %d %d %d\n", num_stages(128, 16), stage_radix(128, 0, 16), stage_radix(128, 1, 16));
     return 0;
 }

@@ -1,7 +1,9 @@
 """Register DFTs of the LDS FFT engine (optimizec_amd/csrc/fft_lds.hpp), checked on the
 host: each radix, including the prime-factor composites 10, 14, 15, 21, 35 that the
-multi-pass path's 210-point lines use, against a direct DFT in long double (forward and
-inverse, max abs error over unit-size inputs), and the radix plan of 210."""
+multi-pass path's 210-point lines use and the 4 x 4 radix 16, against a direct DFT in long
+double (forward and inverse, max abs error over unit-size inputs), and the radix plans:
+210 = 15 x 14, 2048 = 8 x 8 x 8 x 4 (default) or 16 x 16 x 8 (radices up to 16),
+1536 = 8 x 8 x 8 x 3 and 128 = 16 x 8 (radices up to 16)."""
 import os
 import shutil
 import subprocess
@@ -26,6 +28,14 @@ def test_register_dfts_match_direct_dft(tmp_path):
             radices[int(f[0])] = (float(f[1]), float(f[2]))
         elif f and f[0] == "plan210":
             assert [int(x) for x in f[1:]] == [2, 15, 14]
-    assert sorted(radices) == [2, 3, 4, 5, 7, 8, 10, 14, 15, 21, 35]
+        elif f and f[0] == "plan2048":
+            assert [int(x) for x in f[1:]] == [4, 8, 8, 8, 4]
+        elif f and f[0] == "plan2048r16":
+            assert [int(x) for x in f[1:]] == [3, 16, 16, 8]
+        elif f and f[0] == "plan1536r16":
+            assert [int(x) for x in f[1:]] == [4, 8, 8, 8, 3]
+        elif f and f[0] == "plan128r16":
+            assert [int(x) for x in f[1:]] == [2, 16, 8]
+    assert sorted(radices) == [2, 3, 4, 5, 7, 8, 10, 14, 15, 16, 21, 35]
     for r, (fwd, inv) in radices.items():
         assert fwd < 1e-14 * r and inv < 1e-14 * r, (r, fwd, inv)
