@@ -40,10 +40,12 @@ using namespace dnsfft;
 namespace {
 
 // threads per workgroup: the diagnostics' line FFTs (kT) and the step's passes (kTZ: the
-// z-direction passes 1, 2, 6, 7; kTXA: pass 3; kTXB: pass 4; kTXF: pass 5), chosen by
-// timing at N = 29400
+// z-direction passes 1, 2, 6; kTXA: pass 3; kTXB: pass 4; kTXF: pass 5; kTZ7: pass 7),
+// chosen by timing at N = 29400 (pass 7 at 256 threads instead of 512: 7.9 -> 7.4 ms FP64,
+// 5.9 -> 5.8 ms with FP32 intermediates, same box; z tiles of 128 bytes, bz_of, timed
+// against 64 and 256 bytes: both slower in every z pass)
 constexpr int kT = 256;
-constexpr int kTZ = 512, kTXA = 512, kTXB = 256, kTXF = 512;
+constexpr int kTZ = 512, kTXA = 512, kTXB = 256, kTXF = 512, kTZ7 = 256;
 // radix plan of pass 4's L2-point lines: radices up to 16 (N = 8192: 128 = 16 x 8 instead
 // of 8 x 8 x 2, pass 4 1.25 -> 1.23 ms; at N = 16384 and 29400 the plan is the same either way)
 constexpr int kXBR = 16;
@@ -734,7 +736,7 @@ hipError_t launch_m(const BigReq &r)
                                static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs, r.ms);
             break;
         case BIG_ZFWD_B:
-            hipLaunchKernelGGL((k_zfwd_b<TC, M_, kTZ>), dim3(S::L1 * tiles), dim3(kTZ), 0, s, static_cast<const TC *>(r.a),
+            hipLaunchKernelGGL((k_zfwd_b<TC, M_, kTZ7>), dim3(S::L1 * tiles), dim3(kTZ7), 0, s, static_cast<const TC *>(r.a),
                                r.om, r.fn, tw, H, r.hs, r.cn);
             break;
         default:
