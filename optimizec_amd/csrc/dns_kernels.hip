@@ -100,6 +100,20 @@ __device__ __forceinline__ double inv_g_of(int iz, double gz) { return (iz >= 1 
 //             W_M^{-z2 k1} (S[k1] + S[k1+2H] W_3^{-2 z2}), inverse-transformed and written
 //             to rows z = z2 + 3 z1 of V.
 // =====================================================================================
+// Line transforms with the opaque thread index (fft_lines OPQ) where that timed faster
+// (row kernel: FP64 H = 768 0.129 -> 0.115 ms, H = 1024 0.216 -> 0.148 ms; FP32 H = 768,
+// 1024, 1536 0.365 -> 0.238 ms; column kernel: FP64 H = 1024 0.128 -> 0.119 ms); slower
+// or equal at the other sizes (N = 4096 and the ensemble sizes among them)
+template <class TC>
+constexpr bool opq_row(int H)
+{
+    return std::is_same<TC, cd>::value ? (H == 768 || H == 1024) : (H == 768 || H == 1024 || H == 1536);
+}
+template <class TC>
+constexpr bool opq_col(int H)
+{
+    return std::is_same<TC, cd>::value && H == 1024;
+}
 template <class TC, int H, int C, int T, int MINW>
 __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
                                            const TC *__restrict__ G, TC *__restrict__ V,
@@ -161,9 +175,9 @@ __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__rest
     };
     auto lines = [&](auto inv) {  // the 3C H-point lines in LDS
         if constexpr (PF)
-            fft_lines<H, LS, 3 * C, T, decltype(inv)::value>(lds, twl, 1);
+            fft_lines<H, LS, 3 * C, T, decltype(inv)::value, 8, opq_col<TC>(H)>(lds, twl, 1);
         else
-            fft_lines<H, LS, 3 * C, T, decltype(inv)::value>(lds, twT, 3);
+            fft_lines<H, LS, 3 * C, T, decltype(inv)::value, 8, opq_col<TC>(H)>(lds, twT, 3);
     };
 
     // registers: the (lo, hi) pair of each owned (column c, k1): iz = k1 and iz = k1 + H
@@ -421,7 +435,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
         }
         __syncthreads();
-        fft_lines<H, LS, 3, T, true, RXI>(lds, twl, 1);
+        fft_lines<H, LS, 3, T, true, RXI, opq_row<TC>(H)>(lds, twl, 1);
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
@@ -457,7 +471,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
         // A = P0 + i P1, B = P2), then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
         TC *A = lds;
         TC *B = lds + LS;
-        fft_lines<H, LS, 2, T, false>(lds, twl, 1);
+        fft_lines<H, LS, 2, T, false, 8, opq_row<TC>(H)>(lds, twl, 1);
         TC *g = G + (size_t)z * H;
         const RT half = RT(0.5);
         for (int k = tid; k < H; k += T) {
@@ -661,7 +675,7 @@ __global__ __launch_bounds__(T) void k_row_phys(const cd *__restrict__ Va, const
             A[pidx(k1)] = mulc(add(zlo, mulc(zhi, w3)), tw[opaque(x2 * k1)]);
         }
         __syncthreads();
-        fft_lines<H, LS, 1, T, true, 8, cd>(A, tw, 3);
+        fft_lines<H, LS, 1, T, true, 8, false, cd>(A, tw, 3);
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
             const int x1 = tid + j * T;

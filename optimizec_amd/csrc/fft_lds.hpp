@@ -313,7 +313,7 @@ __device__ __forceinline__ void dft_pfa(C *v)
 }
 
 // ---- one Stockham stage over NL lines in LDS -----------------------------------
-template <int L, int LS, int NL, int T, int R, int NS, bool INV, class C>
+template <int L, int LS, int NL, int T, int R, int NS, bool INV, bool OPQ, class C>
 __device__ __forceinline__ void stage(C *buf, const C *__restrict__ tw, int twmul)
 {
     constexpr int LR = L / R;
@@ -321,7 +321,9 @@ __device__ __forceinline__ void stage(C *buf, const C *__restrict__ tw, int twmu
     constexpr int PER = (NB + T - 1) / T;
     constexpr int TWS = L / (NS * R);
     C v[PER][R];
-    const int tid = threadIdx.x;
+    // OPQ: the thread index through an empty asm, so the stage's index arithmetic is not
+    // hoisted out of a caller's loop into registers (fewer VGPRs; faster in some kernels)
+    const int tid = OPQ ? opaque((int)threadIdx.x) : (int)threadIdx.x;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int b = tid + q * T;
@@ -370,12 +372,12 @@ __device__ __forceinline__ void stage(C *buf, const C *__restrict__ tw, int twmu
     __syncthreads();
 }
 
-template <int L, int LS, int NL, int T, bool INV, int S, int MAXR, class C>
+template <int L, int LS, int NL, int T, bool INV, int S, int MAXR, bool OPQ, class C>
 __device__ __forceinline__ void stages_from(C *buf, const C *__restrict__ tw, int twmul)
 {
     if constexpr (S < num_stages(L, MAXR)) {
-        stage<L, LS, NL, T, stage_radix(L, S, MAXR), stage_span(L, S, MAXR), INV, C>(buf, tw, twmul);
-        stages_from<L, LS, NL, T, INV, S + 1, MAXR, C>(buf, tw, twmul);
+        stage<L, LS, NL, T, stage_radix(L, S, MAXR), stage_span(L, S, MAXR), INV, OPQ, C>(buf, tw, twmul);
+        stages_from<L, LS, NL, T, INV, S + 1, MAXR, OPQ, C>(buf, tw, twmul);
     }
 }
 
@@ -384,14 +386,14 @@ __device__ __forceinline__ void stages_from(C *buf, const C *__restrict__ tw, in
 // written) -- the first stage reads after the caller's __syncthreads -- and the
 // engine ends with a barrier, so results are visible to every thread on return.
 // tw may point to global memory or to an LDS table.  MAXR selects the radix plan (8 or 16,
-// see pick_radix).
-template <int L, int LS, int NL, int T, bool INV, int MAXR = 8, class C>
+// see pick_radix); OPQ keeps each stage's index arithmetic inside the caller's loops.
+template <int L, int LS, int NL, int T, bool INV, int MAXR = 8, bool OPQ = false, class C>
 __device__ __forceinline__ void fft_lines(C *buf, const C *__restrict__ tw, int twmul)
 {
     static_assert(smooth7(L), "line length must factor into 2, 3, 5, 7");
     static_assert(LS >= padded_len(L), "line stride must hold the padded line");
     static_assert(MAXR == 8 || MAXR == 16, "radix plans: 8 or 16");
-    stages_from<L, LS, NL, T, INV, 0, MAXR, C>(buf, tw, twmul);
+    stages_from<L, LS, NL, T, INV, 0, MAXR, OPQ, C>(buf, tw, twmul);
 }
 
 }  // namespace dnsfft
