@@ -103,16 +103,18 @@ __device__ __forceinline__ double inv_g_of(int iz, double gz) { return (iz >= 1 
 // Line transforms with the opaque thread index (fft_lines OPQ) where that timed faster
 // (row kernel: FP64 H = 768 0.129 -> 0.115 ms, H = 1024 0.216 -> 0.148 ms; FP32 H = 768,
 // 1024, 1536 0.365 -> 0.238 ms; column kernel: FP64 H = 1024 0.128 -> 0.119 ms); slower
-// or equal at the other sizes (N = 4096 and the ensemble sizes among them)
+// or equal at the ensemble sizes.  FP64 at H = 2048: it brings both kernels under 128
+// VGPRs, so they run 1024 threads per workgroup (row 0.82 -> 0.79-0.81 ms, column 0.55 ->
+// 0.535 ms; at 512 threads the opaque index made them slower)
 template <class TC>
 constexpr bool opq_row(int H)
 {
-    return std::is_same<TC, cd>::value ? (H == 768 || H == 1024) : (H == 768 || H == 1024 || H == 1536);
+    return std::is_same<TC, cd>::value ? (H == 768 || H == 1024 || H == 2048) : (H == 768 || H == 1024 || H == 1536);
 }
 template <class TC>
 constexpr bool opq_col(int H)
 {
-    return std::is_same<TC, cd>::value && H == 1024;
+    return std::is_same<TC, cd>::value && (H == 1024 || H == 2048);
 }
 template <class TC, int H, int C, int T, int MINW>
 __global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
@@ -375,9 +377,6 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
     };
 
     TC vel[3][PO];     // u + i w at x = n2 + 3 x1
-    // radix plan of the inverse lines: 16 x 16 x 8 for the FP64 step at H = 2048 (timed:
-    // 0.84 -> 0.82 ms; the forward lines and the other sizes gain nothing from it)
-    constexpr int RXI = std::is_same<TC, cd>::value && H == 2048 && MODE == 0 ? 16 : 8;
     double cmax = 0.0;
 
     constexpr int NPH = (MODE == 0) ? 2 : 1;
@@ -435,7 +434,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
         }
         __syncthreads();
-        fft_lines<H, LS, 3, T, true, RXI, opq_row<TC>(H)>(lds, twl, 1);
+        fft_lines<H, LS, 3, T, true, 8, opq_row<TC>(H)>(lds, twl, 1);
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
@@ -862,7 +861,7 @@ DNS_CFG(512, 2, 512, 256)
 DNS_CFG(768, 2, 768, 384)
 DNS_CFG(1024, 2, 1024, 512)
 DNS_CFG(1536, 1, 768, 768)
-DNS_CFG(2048, 1, 512, 512)
+DNS_CFG(2048, 1, 1024, 1024)
 #undef DNS_CFG
 
 // Minimum waves per SIMD the compiler budgets VGPRs for.  Forcing two workgroups per
