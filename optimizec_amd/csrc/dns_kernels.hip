@@ -114,10 +114,10 @@ constexpr bool opq_row(int H)
 template <class TC>
 constexpr bool opq_col(int H)
 {
-    return std::is_same<TC, cd>::value && (H == 1024 || H == 2048);
+    return H == 2048 || (std::is_same<TC, cd>::value && H == 1024);
 }
 template <class TC, int H, int C, int T, int MINW>
-__global__ __launch_bounds__(T, MINW) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
                                            const TC *__restrict__ G, TC *__restrict__ V,
                                            const TC *__restrict__ twT, const cd *__restrict__ tw, const ColArgs a,
                                            BatchArgs bt)
@@ -864,13 +864,15 @@ DNS_CFG(1536, 1, 768, 768)
 DNS_CFG(2048, 1, 1024, 1024)
 #undef DNS_CFG
 
-// Minimum waves per SIMD the compiler budgets VGPRs for.  Forcing two workgroups per
-// CU (FP32 lines at H >= 1024 fit twice in LDS) made the kernels spill and run slower
-// on MI355X (row 0.71 -> 0.76 ms, column 0.45 -> 0.55 ms at N=4096), so it stays 1.
+// Minimum waves per SIMD the compiler budgets VGPRs for (amdgpu_waves_per_eu).  Forcing
+// two workgroups per CU (FP32 lines at H >= 1024 fit twice in LDS) made the kernels spill
+// and run slower on MI355X (row 0.71 -> 0.76 ms, column 0.45 -> 0.55 ms at N=4096), except
+// the FP32 column kernel at H = 2048 with the opaque FFT index: 512 threads, two
+// workgroups per CU (4 waves per SIMD, 128 VGPRs, a few spilled), 0.41 -> 0.38 ms.
 template <class TC, int H, int T, int LDS_BYTES>
 constexpr int min_waves()
 {
-    return 1;
+    return (std::is_same<TC, cf>::value && H == 2048 && T == 512 && 2 * LDS_BYTES <= 160 * 1024) ? 4 : 1;
 }
 
 // Launch variants (the ensemble's autotuner times them): the column tile and the
@@ -909,11 +911,12 @@ constexpr bool use_row_groups()
 }
 
 // FP32 transforms at H = 2048 (N = 4096): the lines take half the LDS and the kernels fewer
-// VGPRs, so both run 1024 threads per workgroup (timed: row 0.74 -> 0.70 ms, column
-// 0.50 -> 0.47 ms; results bitwise unchanged -- the same arithmetic per element)
+// VGPRs, so the row pass runs 1024 threads per workgroup (row pairs, k_row_mr) and the
+// column pass 512 threads with two workgroups per CU (min_waves); results bitwise
+// unchanged -- the same arithmetic per element
 struct Cfg32_2048 {
     static constexpr int C = 1;
-    static constexpr int TC = 1024;
+    static constexpr int TC = 512;
     static constexpr int TR = 1024;
 };
 
