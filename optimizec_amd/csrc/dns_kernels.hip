@@ -109,7 +109,7 @@ __device__ __forceinline__ double inv_g_of(int iz, double gz) { return (iz >= 1 
 template <class TC>
 constexpr bool opq_row(int H)
 {
-    return std::is_same<TC, cd>::value ? (H == 768 || H == 1024 || H == 2048) : (H == 768 || H == 1024 || H == 1536);
+    return H == 768 || H == 1024 || H == 2048 || (!std::is_same<TC, cd>::value && H == 1536);
 }
 template <class TC>
 constexpr bool opq_col(int H)
@@ -501,8 +501,8 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
 }
 
 // =====================================================================================
-// Row kernel over row groups (H = 128 .. 512, and FP32 at H = 2048): R consecutive rows
-// per workgroup, MODE 0 only (the step).  V is column-major, so the R rows' values of one mode kx sit side by side (16 R
+// Row kernel over row groups (H = 128 .. 512): R consecutive rows per workgroup, MODE 0
+// only (the step).  V is column-major, so the R rows' values of one mode kx sit side by side (16 R
 // bytes in FP64); lane l of a wave takes row z0 + (l mod 32) mod R and mode slot
 // (l mod 32) / R, so each load instruction reads R rows x 32 / R modes in runs of R values
 // instead of one value per line.  Per row the work is k_row's, line for line (3R lines per
@@ -559,11 +559,8 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
             b[j] = fb[(size_t)k * M];
         }
     };
-    // LDS line of residue n2 of row r: residue-major (all rows in one transform call) or,
-    // at H = 2048 (FP32), row-major with one transform call per row, which halves the
-    // butterflies a thread holds across a stage's barrier (the kernel then fits 128 VGPRs)
-    constexpr bool BYROW = H == 2048;
-    auto line = [](int r, int n2) { return BYROW ? 3 * r + n2 : n2 * R + r; };
+    // LDS line of residue n2 of row r: residue-major, all rows in one transform call
+    auto line = [](int r, int n2) { return n2 * R + r; };
     issue(row, ra0, rb0);
 #pragma unroll
     for (int ph = 0; ph < 2; ++ph) {
@@ -592,12 +589,7 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
         }
         if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
         __syncthreads();
-        if constexpr (BYROW) {
-#pragma unroll
-            for (int rr = 0; rr < R; ++rr) fft_lines<H, LS, 3, T, true>(lds + 3 * rr * LS, twl, 1);
-        } else {
-            fft_lines<H, LS, 3 * R, T, true>(lds, twl, 1);
-        }
+        fft_lines<H, LS, 3 * R, T, true>(lds, twl, 1);
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int e = tid + j * T;
@@ -622,12 +614,7 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
         __syncthreads();
     }
     // x-forward of each row's product, then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
-    if constexpr (BYROW) {
-#pragma unroll
-        for (int rr = 0; rr < R; ++rr) fft_lines<H, LS, 2, T, false>(lds + 3 * rr * LS, twl, 1);
-    } else {
-        fft_lines<H, LS, 2 * R, T, false>(lds, twl, 1);  // lines (r, 0), (r, 1) are 0 .. 2R - 1
-    }
+    fft_lines<H, LS, 2 * R, T, false>(lds, twl, 1);  // lines (r, 0), (r, 1) are 0 .. 2R - 1
     const RT half = RT(0.5);
     for (int e = tid; e < R * H; e += T) {
         const int rr = e / H, k = e - rr * H;
@@ -864,15 +851,15 @@ DNS_CFG(1536, 1, 768, 768)
 DNS_CFG(2048, 1, 1024, 1024)
 #undef DNS_CFG
 
-// Minimum waves per SIMD the compiler budgets VGPRs for (amdgpu_waves_per_eu).  Forcing
-// two workgroups per CU (FP32 lines at H >= 1024 fit twice in LDS) made the kernels spill
-// and run slower on MI355X (row 0.71 -> 0.76 ms, column 0.45 -> 0.55 ms at N=4096), except
-// the FP32 column kernel at H = 2048 with the opaque FFT index: 512 threads, two
-// workgroups per CU (4 waves per SIMD, 128 VGPRs, a few spilled), 0.41 -> 0.38 ms.
+// Minimum waves per SIMD the compiler budgets the column kernel's VGPRs for
+// (amdgpu_waves_per_eu).  Forcing two workgroups per CU (FP32 lines at H >= 1024 fit twice
+// in LDS) made the kernels spill and run slower on MI355X (column 0.45 -> 0.55 ms at
+// N=4096), except with the opaque FFT index: the FP32 column kernel at H = 2048 runs two
+// 1024-thread workgroups per CU (8 waves per SIMD, 64 VGPRs, a few spilled), 0.41 -> 0.37 ms.
 template <class TC, int H, int T, int LDS_BYTES>
-constexpr int min_waves()
+constexpr int col_min_waves()
 {
-    return (std::is_same<TC, cf>::value && H == 2048 && T == 512 && 2 * LDS_BYTES <= 160 * 1024) ? 4 : 1;
+    return (std::is_same<TC, cf>::value && H == 2048 && T == 1024 && 2 * LDS_BYTES <= 160 * 1024) ? 8 : 1;
 }
 
 // Launch variants (the ensemble's autotuner times them): the column tile and the
@@ -893,30 +880,28 @@ hipError_t launch_cfg(const LaunchReq &r);
 
 // rows per workgroup and threads of k_row_mr, timed on the ensemble (192 x N=256 +
 // 64 x N=512): two rows per workgroup beat four and eight (fewer VGPRs and LDS, more
-// workgroups per CU outweigh the longer runs of V values).  FP32 at H = 2048 (N = 4096):
-// two rows, 1024 threads, row pass 0.58 -> 0.50 ms (512 threads: 0.51 ms); two FP64 rows
-// of that length do not fit in LDS.
+// workgroups per CU outweigh the longer runs of V values).  At N = 4096 in FP32, two rows
+// per workgroup (127 KB of LDS, one workgroup per CU) ran 0.50 ms; the one-row kernel with
+// the opaque FFT index (61 VGPRs, two 1024-thread workgroups per CU) runs 0.355 ms.
 template <class TC, int H>
 struct MrCfg {
     static constexpr int R = 2;
-    static constexpr int T = H == 128 ? 128 : H == 2048 ? 1024 : 256;
+    static constexpr int T = H == 128 ? 128 : 256;
 };
 // which row kernel the step's row pass runs: row groups at H = 128 .. 512 (the default row
-// threads) and for FP32 at H = 2048, one row per workgroup otherwise
+// threads), one row per workgroup otherwise
 template <class TC, int H, class Q>
 constexpr bool use_row_groups()
 {
-    if constexpr (H == 2048) return std::is_same<TC, cf>::value;
-    else return H >= 128 && H <= 512 && (H & (H - 1)) == 0 && Q::TR == Cfg<H>::TR;
+    return H >= 128 && H <= 512 && (H & (H - 1)) == 0 && Q::TR == Cfg<H>::TR;
 }
 
-// FP32 transforms at H = 2048 (N = 4096): the lines take half the LDS and the kernels fewer
-// VGPRs, so the row pass runs 1024 threads per workgroup (row pairs, k_row_mr) and the
-// column pass 512 threads with two workgroups per CU (min_waves); results bitwise
-// unchanged -- the same arithmetic per element
+// FP32 transforms at H = 2048 (N = 4096): the lines take half the LDS and, with the opaque
+// FFT index, the kernels fit two 1024-thread workgroups per CU (row 0.50 -> 0.355 ms,
+// column 0.41 -> 0.37 ms); results bitwise unchanged -- the same arithmetic per element
 struct Cfg32_2048 {
     static constexpr int C = 1;
-    static constexpr int TC = 512;
+    static constexpr int TC = 1024;
     static constexpr int TR = 1024;
 };
 
@@ -941,8 +926,8 @@ template <class TC, int H, class Q>
 hipError_t launch_cfg(const LaunchReq &r)
 {
     constexpr int M = 3 * H;
-    constexpr int WC = min_waves<TC, H, Q::TC, 3 * Q::C * padded_len(H) * (int)sizeof(TC)>();
-    constexpr int WR = min_waves<TC, H, Q::TR, 3 * padded_len(H) * (int)sizeof(TC)>();
+    constexpr int WC = col_min_waves<TC, H, Q::TC, 3 * Q::C * padded_len(H) * (int)sizeof(TC)>();
+    constexpr int WR = 1;
     const TC *twT = static_cast<const TC *>(r.twT);
     const unsigned nb = r.batch.count > 1 ? (unsigned)r.batch.count : 1u;
     switch (r.what) {
