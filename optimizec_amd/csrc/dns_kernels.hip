@@ -289,6 +289,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
 #pragma unroll 1
     for (int fi = 0; fi < nf; ++fi) {
         const int f = a.fields[fi];
+        // FP32 with the opaque FFT index: this field's own copy of the thread index, so its
+        // index arithmetic stays in the loop instead of being hoisted into registers (at
+        // N = 4096: 60 VGPRs and no spill, column kernel 0.37 -> 0.33 ms; FP64 is not faster)
+        constexpr bool FOPQ = opq_col<TC>(H) && !std::is_same<TC, cd>::value;
+        const int tid = FOPQ ? opaque((int)threadIdx.x) : (int)threadIdx.x;
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
             const int p = tid + j * T;
@@ -855,7 +860,7 @@ DNS_CFG(2048, 1, 1024, 1024)
 // (amdgpu_waves_per_eu).  Forcing two workgroups per CU (FP32 lines at H >= 1024 fit twice
 // in LDS) made the kernels spill and run slower on MI355X (column 0.45 -> 0.55 ms at
 // N=4096), except with the opaque FFT index: the FP32 column kernel at H = 2048 runs two
-// 1024-thread workgroups per CU (8 waves per SIMD, 64 VGPRs, a few spilled), 0.41 -> 0.37 ms.
+// 1024-thread workgroups per CU (8 waves per SIMD, 64 VGPRs), 0.41 -> 0.33 ms.
 template <class TC, int H, int T, int LDS_BYTES>
 constexpr int col_min_waves()
 {
