@@ -1,7 +1,7 @@
 """FP64 CPU oracle of the subgrid-closure network (test infrastructure only: imported by
 tests/ as the checker of the GPU kernels, never by the product).
 
-Network (north-star configuration, no reference counterpart): four 3x3 convolutions
+Network (the closure configuration, no reference counterpart): four 3x3 convolutions
 with circular padding, 3 -> 32 -> 32 -> 32 -> 1 channels, GELU (erf form) after the
 first three, mean-squared-error loss, Adam (Kingma & Ba 2015, bias-corrected moments).
 

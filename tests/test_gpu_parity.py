@@ -1,6 +1,6 @@
 """GPU parity of the HIP FP64 path against the FP64 CPU oracle, through the C ABI.
 
-Bars (north star): relative L2 vorticity error <= 1e-11 after 100 steps, <= 1e-9 on
+Bars: relative L2 vorticity error <= 1e-11 after 100 steps, <= 1e-9 on
 the energy, enstrophy and dt series; single transforms and single steps agree to
 roundoff (<= 1e-13 relative).  At sizes the oracle cannot follow for long (N=4096),
 a few steps are compared directly and the rest is checked through size-independent
@@ -201,7 +201,7 @@ def test_determinism_and_restart_roundtrip():
 @pytest.mark.parametrize("N", [128, 512, 1024])
 def test_fp32_transforms_against_fp64_oracle(oracle_mod, N):
     """Mixed precision (FP32 transforms, FP64 state, CN/AB2, CFL/dt, E/Z): relative L2
-    vorticity error vs the FP64 oracle <= 1e-5 after 100 steps."""
+    vorticity error vs the FP64 oracle <= 1e-5 after 100 steps (the mixed-precision bar)."""
     steps, update = 100, 20
     s = oa.Solver(N, 1e4, 4.0, 0.1, precision=oa.PREC_FP32)
     s.init_ic(oa.IC_K4EXP, 0)
