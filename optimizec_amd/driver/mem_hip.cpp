@@ -10,7 +10,8 @@
 //
 // Environment: PHASE_TIMING, BENCH_WARMUP_STEPS, RESTART_STEPS, MOV, MOV_DIR (as in
 // mem_cuda.cu:232-243, 1680-1806), plus DNS_IC=pao|k4exp (initial condition; the
-// reference's PAO-k^-3 by default) and DNS_SEED (seed of the k4exp spectrum).  MOV frames
+// reference's PAO-k^-3 by default), DNS_SEED (seed of the k4exp spectrum) and
+// DNS_PREC=fp64|fp32 (fp32: FP32 transforms, FP64 state; FP64 by default).  MOV frames
 // go to $MOV_DIR/hip_<N>_<K0>_<Re>_<CFL>_U<UPDATE>/ (MOV_DIR defaults to ./movies; the
 // reference's default is a home directory of its author and its prefix "cuda_").
 //
@@ -615,7 +616,11 @@ int main(int argc, char **argv)
     r.Re = Re;
     r.CFL = CFL;
     r.adapt = ADAPT_VISC;
-    dns_config cfg{N, Re, K0, CFL, DNS_PREC_FP64, 0, DNS_PATH_AUTO};
+    // DNS_PREC=fp32 (or mixed): FP32 transforms and intermediates, FP64 state, CN/AB2, dt, E/Z
+    const char *prec_env = std::getenv("DNS_PREC");
+    const int prec = (prec_env && (iequals(prec_env, "fp32") || iequals(prec_env, "mixed"))) ? DNS_PREC_FP32 : DNS_PREC_FP64;
+    if (prec == DNS_PREC_FP32) std::printf(" precision: FP32 transforms, FP64 state (DNS_PREC=%s)\n", prec_env);
+    dns_config cfg{N, Re, K0, CFL, prec, 0, DNS_PATH_AUTO};
     dns_status st = dns_create(&cfg, &r.h);
     if (st != DNS_OK) {
         std::fprintf(stderr, "dns_create: %s\n", dns_last_error(nullptr));

@@ -38,6 +38,34 @@ def test_driver_matches_reference_log(tmp_path):
     assert int(fields["SIG"]) == kat["it1"]["SIG"]
 
 
+def _iterations(stdout):
+    rows = {}
+    for ln in stdout.splitlines():
+        if ln.startswith(" ITERATION"):
+            f = dict(tok.split("=") for tok in ln.split()[2:] if "=" in tok)
+            rows[int(ln.split()[1])] = f
+    return rows
+
+
+def test_driver_mixed_precision_knob(tmp_path):
+    """DNS_PREC=fp32 runs the mixed-precision step from the command line: the ITERATION
+    lines follow the FP64 run's (T, DT, CN, CFLM to 1e-6 relative; NEXTDT runs in FP64) and
+    the run announces its precision."""
+    args = ["512", "1e4", "4", "200", "0.1", "50", "0"]
+    env = dict(os.environ, DNS_IC="k4exp", DNS_SEED="2")
+    r64 = subprocess.run([DRIVER, *args], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
+    r32 = subprocess.run([DRIVER, *args], cwd=tmp_path, env=dict(env, DNS_PREC="fp32"), capture_output=True,
+                         text=True, timeout=120)
+    assert r64.returncode == 0 and r32.returncode == 0, r64.stderr + r32.stderr
+    assert "precision: FP32 transforms" in r32.stdout and "precision:" not in r64.stdout
+    a, b = _iterations(r64.stdout), _iterations(r32.stdout)
+    assert sorted(a) == sorted(b) == [1, 50, 100, 150, 200]
+    for it in a:
+        for key in ("T", "DT", "CN", "CFLM"):
+            x, y = float(a[it][key]), float(b[it][key])
+            assert abs(y - x) <= 1e-6 * abs(x), (it, key, x, y)
+
+
 def test_driver_restart_roundtrip(tmp_path):
     """SIGRTMIN writes output_..._restart/restart.bin (v2, 8-byte reals); RESTART continues
     to the same state as an uninterrupted run (bitwise: final T line)."""
