@@ -49,6 +49,15 @@ constexpr int kTZ = 512, kTXA = 512, kTXB = 256, kTXF = 512, kTZ7 = 256;
 // radix plan of pass 4's L2-point lines: radices up to 16 (N = 8192: 128 = 16 x 8 instead
 // of 8 x 8 x 2, pass 4 1.25 -> 1.23 ms; at N = 16384 and 29400 the plan is the same either way)
 constexpr int kXBR = 16;
+// passes whose line transforms take the opaque thread index (fft_lines OPQ), where that
+// timed faster: passes 4 and 7 with FP32 intermediates at N = 29400 (pass 4 11.6 -> 11.25 ms,
+// pass 7 7.3 -> 6.3-7.1 ms; results bitwise unchanged); FP64, the other passes and the
+// smaller grids (pass 4 at N = 16384: 2.97 -> 3.12 ms) are not faster with it
+template <class TC, int M_>
+constexpr bool big_opq(int pass)
+{
+    return !std::is_same<TC, cd>::value && M_ == 44100 && (pass == 4 || pass == 7);
+}
 
 __device__ __forceinline__ double gam(int iz, int N) { return (iz < (N >> 1)) ? double(iz) : double(iz - N); }
 
@@ -412,7 +421,7 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
     tf2.put(w2);
     tfb.put(wb);
     __syncthreads();
-    fft_lines<L2, LS, G * C, T, true, kXBR>(lds, w2, 1);
+    fft_lines<L2, LS, G * C, T, true, kXBR, big_opq<TC, M_>(4)>(lds, w2, 1);
     if (MODE == 1) {
         double m = 0.0;
         for (int e = threadIdx.x; e < 2 * C * L2; e += T) {
@@ -442,7 +451,7 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
         lds[c * LS + pidx(k2)] = TC{RT(a0.x * b0.x + a0.y * b0.y), RT(a1.x * b1.x + a1.y * b1.y)};
     }
     __syncthreads();
-    fft_lines<L2, LS, C, T, false, kXBR>(lds, w2, 1);
+    fft_lines<L2, LS, C, T, false, kXBR, big_opq<TC, M_>(4)>(lds, w2, 1);
     for (int e = threadIdx.x; e < C * L2; e += T) {
         const int c = e / L2, m2 = e % L2;
         const int k1 = k10 + c;
@@ -569,7 +578,7 @@ __global__ __launch_bounds__(T) void k_zfwd_b(const TC *__restrict__ W, cd *__re
     }
     tf2.put(w2);
     __syncthreads();
-    fft_lines<L2, LS, BZ, T, false>(lds, w2, 1);
+    fft_lines<L2, LS, BZ, T, false, 8, big_opq<TC, M_>(7)>(lds, w2, 1);
     for (int e = threadIdx.x; e < BZ * L2; e += T) {
         const int c = e % BZ, k2 = e / BZ;
         const int kx = kx0 + c, kz = k1 + L1 * k2;
