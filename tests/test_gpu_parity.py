@@ -57,7 +57,7 @@ def test_physical_fields(oracle_mod, N):
     s.close()
 
 
-@pytest.mark.parametrize("N", [16, 64, 256, 512, 2048])
+@pytest.mark.parametrize("N", [16, 64, 256, 512, 1536, 2048, 3072])
 def test_start_and_single_step(oracle_mod, N):
     s, o = _pair(oracle_mod, N, seed=1)
     s.start()
@@ -126,11 +126,13 @@ def test_4096_short_parity_and_invariants(oracle_mod):
     s.close()
 
 
-def test_4096_fp32_transforms_against_fp64():
-    """N=4096 with FP32 transforms (its row pass runs two rows per workgroup) against the
-    FP64 path, which the test above pins to the oracle: relative L2 vorticity error after
-    100 steps <= 1e-5, and above 1e-12 (the transforms really ran in FP32)."""
-    N, steps, update = 4096, 100, 20
+@pytest.mark.parametrize("N", [1536, 3072, 4096])
+def test_large_fp32_transforms_against_fp64(N):
+    """FP32 transforms at N = 1536, 3072, 4096 (kernels with the opaque FFT index, two
+    workgroups per CU at 4096) against the FP64 path, which the single-step and N=4096 tests
+    pin to the oracle: relative L2 vorticity error after 100 steps <= 1e-5, and above 1e-12
+    (the transforms really ran in FP32)."""
+    steps, update = 100, 20
     runs = []
     for prec in (oa.PREC_FP64, oa.PREC_FP32):
         print(f"N={N} precision={prec}: {steps} steps", flush=True)
