@@ -46,6 +46,14 @@ namespace {
 // against 64 and 256 bytes: both slower in every z pass)
 constexpr int kT = 256;
 constexpr int kTZ = 512, kTXA = 512, kTXB = 256, kTXF = 512, kTZ7 = 256;
+// pass 3 with FP32 intermediates at N = 29400 runs 1024 threads (13.8 -> 12.6 ms; at
+// N = 8192 and 16384 it is slower, and the other FP32 passes at 256 or 1024 threads were
+// slower or equal)
+template <class TC, int M_>
+constexpr int txa_of()
+{
+    return (!std::is_same<TC, cd>::value && M_ == 44100) ? 1024 : kTXA;
+}
 // radix plan of pass 4's L2-point lines: radices up to 16 (N = 8192: 128 = 16 x 8 instead
 // of 8 x 8 x 2, pass 4 1.25 -> 1.23 ms; at N = 16384 and 29400 the plan is the same either way)
 constexpr int kXBR = 16;
@@ -724,7 +732,7 @@ hipError_t launch_m(const BigReq &r)
             break;
         case BIG_XINV_A: {
             constexpr int NPT = (S::L2 / 2 - 1 + BZ - 1) / BZ;
-            hipLaunchKernelGGL((k_xinv_a<TC, M_, kTXA>), dim3((NPT + 1) * M_ * (unsigned)r.planes), dim3(kTXA), 0, s,
+            hipLaunchKernelGGL((k_xinv_a<TC, M_, txa_of<TC, M_>()>), dim3((NPT + 1) * M_ * (unsigned)r.planes), dim3(txa_of<TC, M_>()), 0, s,
                                static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs, r.ms);
             break;
         }
