@@ -150,13 +150,17 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
     // without any load inside them waiting behind those (vector loads complete in order).
     constexpr bool PF = H <= 512;
     constexpr bool F64 = std::is_same<TC, cd>::value;
-    __shared__ TC twl[PF ? H : 1];
-    __shared__ cd twd[PF && !F64 ? H : 1];
+    // TAB: the twiddle tables in LDS -- at the small H, and for FP64 at H = 2048, where the
+    // table fits beside the three lines without costing a workgroup per CU (one fits anyway;
+    // with the field loop's opaque index below, 0.534 -> 0.525 ms)
+    constexpr bool TAB = PF || (F64 && H == 2048);
+    __shared__ TC twl[TAB ? H : 1];
+    __shared__ cd twd[TAB && !F64 ? H : 1];
 
     const int tid = threadIdx.x;
     const int kx0 = xcd_tile(blockIdx.x, gridDim.x) * C;
     const cd wm1 = tw[1], wm2 = tw[2];
-    if constexpr (PF) {
+    if constexpr (TAB) {
         for (int j = tid; j < H; j += T) {
             twl[j] = twT[3 * j];
             if constexpr (!F64) twd[j] = tw[3 * j];
@@ -164,9 +168,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
         __syncthreads();
     }
     const cd *tabd = nullptr;
-    if constexpr (PF) tabd = F64 ? reinterpret_cast<const cd *>(twl) : twd;
+    if constexpr (TAB) tabd = F64 ? reinterpret_cast<const cd *>(twl) : twd;
     auto wM = [&](int e) -> cd {  // W_M^e, 0 <= e < 2H
-        if constexpr (PF) {
+        if constexpr (TAB) {
             const int q3 = e / 3, r3 = e - 3 * q3;
             const cd t = tabd[q3];
             const cd m = mul(t, cd{r3 == 1 ? wm1.x : wm2.x, r3 == 1 ? wm1.y : wm2.y});
@@ -176,7 +180,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
         }
     };
     auto lines = [&](auto inv) {  // the 3C H-point lines in LDS
-        if constexpr (PF)
+        if constexpr (TAB)
             fft_lines<H, LS, 3 * C, T, decltype(inv)::value, 8, opq_col<TC>(H)>(lds, twl, 1);
         else
             fft_lines<H, LS, 3 * C, T, decltype(inv)::value, 8, opq_col<TC>(H)>(lds, twT, 3);
@@ -289,10 +293,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
 #pragma unroll 1
     for (int fi = 0; fi < nf; ++fi) {
         const int f = a.fields[fi];
-        // FP32 with the opaque FFT index: this field's own copy of the thread index, so its
-        // index arithmetic stays in the loop instead of being hoisted into registers (at
-        // N = 4096: 60 VGPRs and no spill, column kernel 0.37 -> 0.33 ms; FP64 is not faster)
-        constexpr bool FOPQ = opq_col<TC>(H) && !std::is_same<TC, cd>::value;
+        // this field's own copy of the thread index where the transforms take the opaque one
+        // (FP32, and FP64 at H = 2048), so its index arithmetic stays in the loop instead of
+        // being hoisted into registers (FP32 at N = 4096: 60 VGPRs and no spill, column
+        // kernel 0.37 -> 0.33 ms; FP64 at H = 1024 is not faster with it)
+        constexpr bool FOPQ = opq_col<TC>(H) && (!std::is_same<TC, cd>::value || H == 2048);
         const int tid = FOPQ ? opaque((int)threadIdx.x) : (int)threadIdx.x;
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
