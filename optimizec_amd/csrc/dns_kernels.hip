@@ -154,16 +154,19 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
     // table fits beside the three lines without costing a workgroup per CU (one fits anyway;
     // with the field loop's opaque index below, 0.534 -> 0.525 ms)
     constexpr bool TAB = PF || (F64 && H == 2048);
-    __shared__ TC twl[TAB ? H : 1];
+    // TABL: the H-point lines' table W_H^j in LDS; FP32 at H = 2048 takes it alone (16 KB:
+    // two workgroups per CU still fit; 0.326 -> 0.321 ms), its residue factors stay global
+    constexpr bool TABL = TAB || (!F64 && H == 2048);
+    __shared__ TC twl[TABL ? H : 1];
     __shared__ cd twd[TAB && !F64 ? H : 1];
 
     const int tid = threadIdx.x;
     const int kx0 = xcd_tile(blockIdx.x, gridDim.x) * C;
     const cd wm1 = tw[1], wm2 = tw[2];
-    if constexpr (TAB) {
+    if constexpr (TABL) {
         for (int j = tid; j < H; j += T) {
             twl[j] = twT[3 * j];
-            if constexpr (!F64) twd[j] = tw[3 * j];
+            if constexpr (!F64 && TAB) twd[j] = tw[3 * j];
         }
         __syncthreads();
     }
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
         }
     };
     auto lines = [&](auto inv) {  // the 3C H-point lines in LDS
-        if constexpr (TAB)
+        if constexpr (TABL)
             fft_lines<H, LS, 3 * C, T, decltype(inv)::value, 8, opq_col<TC>(H)>(lds, twl, 1);
         else
             fft_lines<H, LS, 3 * C, T, decltype(inv)::value, 8, opq_col<TC>(H)>(lds, twT, 3);
@@ -865,7 +868,8 @@ DNS_CFG(2048, 1, 1024, 1024)
 // (amdgpu_waves_per_eu).  Forcing two workgroups per CU (FP32 lines at H >= 1024 fit twice
 // in LDS) made the kernels spill and run slower on MI355X (column 0.45 -> 0.55 ms at
 // N=4096), except with the opaque FFT index: the FP32 column kernel at H = 2048 runs two
-// 1024-thread workgroups per CU (8 waves per SIMD, 64 VGPRs), 0.41 -> 0.33 ms.
+// 1024-thread workgroups per CU (8 waves per SIMD, 64 VGPRs, 72 KB of LDS each), 0.41 ->
+// 0.32 ms.
 template <class TC, int H, int T, int LDS_BYTES>
 constexpr int col_min_waves()
 {
@@ -936,7 +940,8 @@ template <class TC, int H, class Q>
 hipError_t launch_cfg(const LaunchReq &r)
 {
     constexpr int M = 3 * H;
-    constexpr int WC = col_min_waves<TC, H, Q::TC, 3 * Q::C * padded_len(H) * (int)sizeof(TC)>();
+    // the column kernel's LDS: its 3C lines and the H-point lines' twiddle table
+    constexpr int WC = col_min_waves<TC, H, Q::TC, (3 * Q::C * padded_len(H) + H) * (int)sizeof(TC)>();
     constexpr int WR = 1;
     const TC *twT = static_cast<const TC *>(r.twT);
     const unsigned nb = r.batch.count > 1 ? (unsigned)r.batch.count : 1u;
