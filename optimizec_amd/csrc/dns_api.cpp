@@ -86,6 +86,9 @@ struct dns_handle {
     size_t bytes = 0;
     bool have_ic = false;
     bool v_valid = false;
+    // multi-pass, FP64: pass 3's plane 0 (u + i w rows, x pass A) of the current om is in
+    // buffer A, left there by the NEXTDT CFL; the next step transforms only the gradient pair
+    bool x0_valid = false;
     std::string err;
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -228,16 +231,47 @@ dns_status big_zinv(dns_handle *h, bool fp64, const int *fields, int pairs, bool
 dns_status big_step(dns_handle *h, double dt)
 {
     static const int f4[4] = {DNS_F_U, DNS_F_W, DNS_F_OX, DNS_F_OZ};
-    dns_status st = big_zinv(h, false, f4, 2, true);
-    if (st != DNS_OK) return st;
+    static const int fg[2] = {DNS_F_OX, DNS_F_OZ};
+    const bool reuse = h->x0_valid && !h->fp32;
+    h->x0_valid = false;
+    dns_status st;
     BigReq r{};
     r.fp32 = h->fp32;
     r.tw = h->fp32 ? h->twT : h->tw;
-    r.what = BIG_XINV_A;
-    r.planes = 2;
-    r.a = h->BB;
-    r.b = h->BA;
-    if ((st = big_launch(h, KT_BIG_XINV_A, r)) != DNS_OK) return st;
+    if (reuse) {
+        // plane 0 of pass 3 (u + i w) is in buffer A from the NEXTDT on this om (same
+        // kernels, same arithmetic): passes 1-3 for the gradient pair only, its z-pass-A
+        // output in buffer A past plane 0, its rows at their place (fields 2, 3) in buffer B,
+        // its plane 1 of pass 3 at its place in buffer A.  Untimed: half-size launches.
+        char *ba = static_cast<char *>(h->BA), *bb = static_cast<char *>(h->BB);
+        const size_t e = sizeof(cd), plane = (size_t)h->M * h->ms * e, field = (size_t)h->M * h->hs * e;
+        BigReq z{};
+        z.what = BIG_ZINV_A;
+        z.fp32 = 0;
+        z.tw = h->tw;
+        z.om = h->om;
+        z.b = ba + plane;
+        z.fields.pairs = 1;
+        z.fields.f[0] = fg[0];
+        z.fields.f[1] = fg[1];
+        if ((st = big_launch(h, -1, z)) != DNS_OK) return st;
+        z.what = BIG_ZINV_B;
+        z.a = ba + plane;
+        z.b = bb + 2 * field;
+        if ((st = big_launch(h, -1, z)) != DNS_OK) return st;
+        r.what = BIG_XINV_A;
+        r.planes = 1;
+        r.a = bb + 2 * field;
+        r.b = ba + plane;
+        if ((st = big_launch(h, -1, r)) != DNS_OK) return st;
+    } else {
+        if ((st = big_zinv(h, false, f4, 2, true)) != DNS_OK) return st;
+        r.what = BIG_XINV_A;
+        r.planes = 2;
+        r.a = h->BB;
+        r.b = h->BA;
+        if ((st = big_launch(h, KT_BIG_XINV_A, r)) != DNS_OK) return st;
+    }
     r.what = BIG_XINV_B;
     r.a = h->BA;
     r.b = h->BB;
@@ -265,6 +299,7 @@ dns_status big_step(dns_handle *h, double dt)
 // fields on rows (Z in BB), rows R = a (+ i b) in BA, inverse line FFTs (scratch BB)
 dns_status big_field(dns_handle *h, int field)
 {
+    h->x0_valid = false;  // buffers A and B are reused below
     int f2[2];
     bool two = false;
     switch (field) {
@@ -335,6 +370,7 @@ dns_status big_cflm(dns_handle *h, double *cflm)
     double m = 0.0;
     for (double x : v) m = x > m ? x : m;
     *cflm = m;
+    h->x0_valid = !h->fp32;  // plane 0 of pass 3 for this om stays in buffer A
     return DNS_OK;
 }
 
@@ -490,6 +526,7 @@ dns_status compute_cflm(dns_handle *h, double *cflm)
 dns_status upload_spec(dns_handle *h, const double *host, cd *dst)
 {
     const size_t n = (size_t)h->N * h->H;
+    h->x0_valid = false;  // the state changes
     if (h->big) {  // the multi-pass state is in the host layout
         HIPCHK(h, hipMemcpyAsync(dst, host, sizeof(cd) * n, hipMemcpyHostToDevice, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));

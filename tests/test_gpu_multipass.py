@@ -186,3 +186,30 @@ def test_29400_low_mode_evolution_matches_oracle_128(oracle_mod):
     assert abs(E / Eo - 1) < 1e-12 and abs(Z / Zo - 1) < 1e-12
     s.close()
     o.close()
+
+
+@pytest.mark.parametrize("N", [256, 1024])
+def test_multipass_step_after_nextdt_reuses_plane0(N):
+    """After a NEXTDT (FP64), the next step takes the velocity rows that the CFL pass left in
+    its buffer and transforms only the gradient pair.  A diagnostic between NEXTDT and the step
+    reuses that buffer, so the step then transforms both pairs: the two runs agree bit for bit."""
+    print(f"multipass N={N}: step after NEXTDT with and without the reused rows", flush=True)
+    states = []
+    for diag in (False, True):
+        s = oa.Solver(N, 1e4, 4.0, 0.1, path=MP)
+        s.init_ic(oa.IC_K4EXP, 5)
+        s.start()
+        if diag:
+            s.display_sigma()
+        for it in range(1, 13):
+            s.step(1)
+            if it % 4 == 0:
+                s.next_dt()
+                if diag:
+                    s.display_sigma()
+        states.append((s.om2(), s.fnm1(), s.scalars()))
+        s.close()
+    (a, fa, sa), (b, fb, sb) = states
+    assert np.array_equal(a, b) and np.array_equal(fa, fb)
+    for f in ("it", "t", "dt", "cn", "cnm1", "cflm"):
+        assert getattr(sa, f) == getattr(sb, f), f
