@@ -158,7 +158,9 @@ int dns_ic_pao_host(int N, double Re, double K0, double *om2, char *log, int log
 dns_status dns_ic_k4exp_host(int N, double K0, uint64_t seed, double *om2);
 
 /* ---- measurement ---------------------------------------------------------- */
-/* Enables HIP-event timing of every hot-kernel launch on the handle's stream. */
+/* HIP-event timing of hot-kernel launches on the handle's stream: enabled <= 0 turns it
+ * off, k >= 1 times the 1st, (k+1)-th, (2k+1)-th ... launch of each kernel (1 = every
+ * launch); a pair of events costs a few microseconds of stream time. */
 dns_status dns_set_kernel_timing(dns_handle *h, int enabled);
 /* Per-kernel totals since the last reset: names[i] (static strings), launches,
  * total milliseconds, algorithmic bytes per launch.  *n is capacity in, count out. */

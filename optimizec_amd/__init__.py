@@ -313,8 +313,9 @@ class Solver:
         self._check(load_library().dns_load_restart(self._h, path.encode()))
 
     # -- measurement
-    def set_kernel_timing(self, on: bool) -> None:
-        self._check(load_library().dns_set_kernel_timing(self._h, 1 if on else 0))
+    def set_kernel_timing(self, on: bool, every: int = 1) -> None:
+        """Times the 1st, (every+1)-th, ... launch of each hot kernel with HIP events."""
+        self._check(load_library().dns_set_kernel_timing(self._h, max(1, int(every)) if on else 0))
 
     def reset_kernel_timing(self) -> None:
         self._check(load_library().dns_reset_kernel_timing(self._h))

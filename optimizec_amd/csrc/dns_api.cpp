@@ -90,7 +90,8 @@ struct dns_handle {
     // buffer A, left there by the NEXTDT CFL; the next step transforms only the gradient pair
     bool x0_valid = false;
     std::string err;
-    bool timing = false;
+    int timing_every = 0;  // 0: off; k: every k-th launch of each timed kernel
+    int64_t seen[KT_COUNT] = {};
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> event_pool;
     int64_t launches[KT_COUNT] = {};
@@ -166,7 +167,7 @@ void drain_timing(dns_handle *h)
 TimedLaunch timing_begin(dns_handle *h, int timed_kernel)
 {
     TimedLaunch tl{timed_kernel, nullptr, nullptr};
-    if (h->timing && timed_kernel >= 0) {
+    if (h->timing_every > 0 && timed_kernel >= 0 && h->seen[timed_kernel]++ % h->timing_every == 0) {
         if (h->pending.size() > 4096) drain_timing(h);
         tl.a = take_event(h);
         tl.b = take_event(h);
@@ -1192,7 +1193,8 @@ extern "C" dns_status dns_set_kernel_timing(dns_handle *h, int enabled)
 {
     if (!h) return DNS_E_ARG;
     drain_timing(h);
-    h->timing = enabled != 0;
+    h->timing_every = enabled > 0 ? enabled : 0;
+    for (int i = 0; i < KT_COUNT; ++i) h->seen[i] = 0;
     return DNS_OK;
 }
 
@@ -1203,6 +1205,7 @@ extern "C" dns_status dns_reset_kernel_timing(dns_handle *h)
     for (int i = 0; i < KT_COUNT; ++i) {
         h->launches[i] = 0;
         h->total_ms[i] = 0.0;
+        h->seen[i] = 0;
     }
     return DNS_OK;
 }
