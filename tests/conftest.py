@@ -1,7 +1,7 @@
 """Shared pytest configuration.
 
 Markers:
-  gpu  -- needs an MI355X (HIP device); run with ``-m gpu`` on the GPU box.
+  gpu  -- needs an MI355X (HIP device); run with ``-m gpu`` on a machine that has one.
   slow -- CPU tests that take tens of seconds (still part of the default run).
 """
 import os
