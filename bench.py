@@ -22,8 +22,8 @@ and compares a coarse N=64 run with and without it against the 8x-filtered DNS a
 (Re=1e5, K0=8, t=0.6); the full N=1024 vs N=8192 comparison is run by
 optimizec_amd/closure_eval.py.
 
-Prints one JSON line (see the keys below).  ``--gpus`` above 1 is refused: this build
-runs on one GPU.
+Prints one JSON line (see the keys below).  ``--gpus`` above 1 is refused: the solver
+runs one state per GPU, on one GPU.
 """
 from __future__ import annotations
 
@@ -229,7 +229,7 @@ def main() -> int:
     ap.add_argument("--no-closure", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the solver has no "
               "multi-GPU path.", file=sys.stderr)
         return 2
 

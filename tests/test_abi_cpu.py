@@ -112,3 +112,15 @@ def test_ensemble_create_fails_loudly_without_device():
     with pytest.raises(oa.DnsError) as ei:
         oa.Ensemble([(256, 1e3, 4.0, 0.1, 0)])
     assert ei.value.status == 3
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs one GPU; --gpus above 1 exits with status 2 and says why, before any
+    device work."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "4"],
+                       capture_output=True, text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
