@@ -167,6 +167,27 @@ dns_status dns_set_kernel_timing(dns_handle *h, int enabled);
 dns_status dns_kernel_timing(dns_handle *h, const char **names, int64_t *launches,
                              double *total_ms, double *bytes_per_launch, int *n);
 dns_status dns_reset_kernel_timing(dns_handle *h);
+/* ---- allocation-free step --------------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP stream capture, tight serving loops, harnesses that forbid hipMalloc):
+ * nsteps iterations of the dns_step loop body (FP64 fused kernels, bitwise the results of
+ * dns_step on a handle holding the same state and scalars) on caller-owned device arrays
+ * om2, fnm1 in the reference's device layout [iz][ix] (N x N/2 complex, interleaved
+ * doubles; cuda_dns.h:75-76), enqueued on `stream` (a hipStream_t; NULL = the null
+ * stream) of the current device.  Reads N, visc, dt, cn, cnm1, t, it of *sc and updates
+ * it, t, cnm1 as dns_step does; dt must already be set (dns_start / dns_next_dt on a
+ * handle, or the caller's own NEXTDT).  N must be one of 16, 32, 64, 128, 256, 384,
+ * 512 (DNS_NOALLOC_MAX_N): larger N returns DNS_E_ARG.  All work buffers live in one
+ * file-scope device array of DNS_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends and shared by every call on that GPU: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialized by the caller.  The first call
+ * on a device does a one-time set-up (symbol lookup and a 33 KB synchronous twiddle-table
+ * copy): make it outside any stream capture.  No hipMalloc, hipFree or allocating call
+ * is made.  Errors: dns_last_error(NULL). */
+#define DNS_NOALLOC_MAX_N 512
+#define DNS_NOALLOC_SCRATCH_BYTES (20u * 1024u * 1024u)
+dns_status dns_step_noalloc(dns_scalars *sc, double *om2, double *fnm1, int nsteps, void *stream);
+
 /* Blocks until all queued work of the handle is done. */
 dns_status dns_synchronize(dns_handle *h);
 /* Device bytes allocated by the handle. */

@@ -101,6 +101,7 @@ SIGNATURES = {
     "dns_reset_kernel_timing": (ctypes.c_int, [_P]),
     "dns_synchronize": (ctypes.c_int, [_P]),
     "dns_device_bytes": (ctypes.c_size_t, [_P]),
+    "dns_step_noalloc": (ctypes.c_int, [ctypes.POINTER(Scalars), _P, _P, ctypes.c_int, _P]),
     # batched ensemble (include/dns_ensemble.h)
     "dns_ens_create": (ctypes.c_int, [ctypes.POINTER(EnsConfig), ctypes.POINTER(_P)]),
     "dns_ens_destroy": (None, [_P]),

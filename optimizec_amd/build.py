@@ -44,7 +44,8 @@ def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     procs = []
-    for src in ("dns_kernels.hip", "dns_big.hip", "dns_ens_kernels.hip", "cnn_kernels.hip", "closure_kernels.hip"):
+    for src in ("dns_kernels.hip", "dns_big.hip", "dns_ens_kernels.hip", "cnn_kernels.hip", "closure_kernels.hip",
+                "dns_static.hip"):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         print("+", " ".join(cmd), flush=True)

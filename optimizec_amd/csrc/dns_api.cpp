@@ -56,6 +56,18 @@ const char *const kKernelNames[KT_COUNT] = {"k_row",      "k_col",      "k_row_c
 
 }  // namespace
 
+// W_M^t = e^{-2 pi i t / M} for t in [0, M), evaluated in long double and rounded once
+void dns_host_twiddles(int M, cd *out)
+{
+    for (int t = 0; t < M; ++t) {
+        const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)M;
+        out[t] = cd{(double)cosl(a), -(double)sinl(a)};
+    }
+}
+
+// error text of a call that has no handle (dns_last_error(NULL))
+void dns_note_error(const char *msg) { g_create_error = msg ? msg : ""; }
+
 struct dns_handle {
     int N = 0, M = 0, H = 0, NK = 0;
     int device = 0;
@@ -701,12 +713,8 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
         DNS_OK)
         return bail(st);
 
-    // W_M^t = e^{-2 pi i t / M}, evaluated in long double and rounded once
     std::vector<cd> tw(h->M);
-    for (int t = 0; t < h->M; ++t) {
-        const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)h->M;
-        tw[t] = cd{(double)cosl(a), -(double)sinl(a)};
-    }
+    dns_host_twiddles(h->M, tw.data());
     if ((e = hipMemcpy(h->tw, tw.data(), sizeof(cd) * h->M, hipMemcpyHostToDevice)) != hipSuccess)
         return bail(fail(h, DNS_E_HIP, "twiddle upload: %s", hipGetErrorString(e)));
     h->twT = h->tw;

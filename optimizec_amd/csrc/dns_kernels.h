@@ -78,3 +78,9 @@ hipError_t dns_launch_minmax(const double *f, size_t n, double *out, int blocks,
 hipError_t dns_launch_pixels(const double *f, size_t n, double mn, double rng, int is_const,
                              unsigned long long *out, int blocks, hipStream_t s);
 hipError_t dns_launch_transpose(const dnsfft::cd *in, dnsfft::cd *out, int rows, int cols, hipStream_t s);
+
+// W_M^t = e^{-2 pi i t / M}, t in [0, M), in long double rounded once (host; dns_api.cpp):
+// every path of the library uses this one table
+void dns_host_twiddles(int M, dnsfft::cd *out);
+// sets the text dns_last_error(NULL) returns (calls without a handle)
+void dns_note_error(const char *msg);
