@@ -37,10 +37,15 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 UPDATE = 100
-# The N=4096 legs time each hot kernel with a pair of HIP events on every TIMING_EVERY-th launch
-# in the timed region: a pair costs about 7 us of stream time, so timing every launch of the
-# two kernels would add about 14 us to a 0.67 ms step.
-TIMING_EVERY = 10
+# The N=4096 legs time each hot kernel with a pair of HIP events on every k-th launch in the
+# timed region, k = max(1, steps // TIMING_SAMPLES), so each kernel gets at least
+# TIMING_SAMPLES timed launches whatever --steps is: a pair costs about 7 us of stream time,
+# so timing every launch of the two kernels would add about 14 us to a 0.67 ms step.
+TIMING_SAMPLES = 10
+# algorithmic HBM bytes of one FP64 step: N^2 (2.5 c_s + 7.5 c_i) with c_s = c_i = 16 B
+# (inverse z-pass reads om and writes 4 fields, the row pass reads them and writes one,
+# the forward z-pass + CN/AB2 reads it and om, fn and writes om, fn)
+STEP_BYTES_PER_N2 = 160.0
 
 WORKLOAD = {"N": 4096, "Re": 1.0e5, "K0": 4.0, "CFL": 0.1, "seed": 0}
 BIG = {"N": 29400, "Re": 1.0e7, "K0": 4.0, "CFL": 0.1, "seed": 0, "steps": 41, "update": 20}
@@ -58,28 +63,76 @@ def _traffic_from_profile(kernel: str, N: int):
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(N: int, seconds: float = 15.0):
-    """The FP64 CPU oracle (advective form, OpenMP) on a bounded sample of the same
-    workload: init + start, then as many steps as fit in about `seconds`."""
+def host_facts() -> dict:
+    """CPU model, core counts, OpenMP setting and free memory of this host."""
+    facts = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+             "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        with open("/proc/cpuinfo", encoding="utf-8") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    facts["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        with open("/proc/meminfo", encoding="utf-8") as f:
+            for line in f:
+                if line.startswith("MemAvailable:"):
+                    facts["MemAvailable_GB"] = round(int(line.split()[1]) * 1024 / 1e9, 1)
+                    break
+    except OSError:
+        pass
+    return facts
+
+
+def oracle_host_bytes(N: int) -> float:
+    """Host bytes the FP64 oracle holds during one advective step at base grid N: om2, fnm1,
+    UC (3 M x NK complex), UR (3 M^2 real), and the step's 4 spectra, 4 physical planes and
+    two M x NK complex planes."""
+    H, M, NK = N // 2, 3 * N // 2, 3 * N // 4 + 1
+    nc = N * H
+    return 16.0 * (2 * nc + 3 * M * NK + 4 * nc + 2 * M * NK) + 8.0 * (3 * M * M + 4 * M * M)
+
+
+def cpu_baseline(N: int, steps: int, gpu: dict):
+    """The FP64 CPU oracle (advective form, OpenMP on this host's cores) on the same
+    workload: init, NEXTDT, steps 1..`steps` with NEXTDT on the reference cadence; the
+    median wall time of a step's loop body, and the step-`steps` parity checkpoint of the
+    GPU runs (`gpu`: om2 at step 0 and at step `steps`, FP64 and mixed precision)."""
+    import numpy as np
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # noqa: E402  (checker only: the CPU baseline leg)
 
-    threads = min(16, os.cpu_count() or 1)
+    facts = host_facts()
+    threads = int(facts["OMP_NUM_THREADS"] or facts["affinity_cpus"] or 1)
     oracle.set_threads(threads)
     o = oracle.Oracle(N, WORKLOAD["Re"], WORKLOAD["K0"], WORKLOAD["CFL"])
     o.init_config(WORKLOAD["seed"])
     o.start(form=1)
-    times = []
-    t_end = time.perf_counter() + seconds
-    while time.perf_counter() < t_end and len(times) < 50:
-        t0 = time.perf_counter()
-        o.one_step(form=1)
-        times.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    rows, secs = o.run_timed(steps, UPDATE, form=1)
+    wall = time.perf_counter() - t0
+    om = o.om2()
     o.close()
-    times.sort()
-    med = times[len(times) // 2]
-    return {"value": round(1e3 * med, 3), "unit": "ms/step", "cores": threads, "kind": "port",
-            "sample": f"N={N} FP64 advective-form oracle, {len(times)} steps after init, median"}
+    change = float(np.linalg.norm(om - gpu["om0"]))
+    parity = {"step": steps}
+    for key, name in (("om64", "gpu_fp64"), ("om32", "gpu_mixed_fp32")):
+        if gpu.get(key) is not None:
+            d = float(np.linalg.norm(gpu[key] - om))
+            parity[f"rel_l2_vorticity_{name}_vs_oracle"] = d / float(np.linalg.norm(om))
+            parity[f"rel_l2_vorticity_{name}_vs_oracle_per_change"] = d / change
+    big_need = oracle_host_bytes(BIG["N"]) / 1e9
+    return {"value": round(1e3 * float(np.median(secs)), 3), "unit": "ms/step", "cores": threads, "kind": "port",
+            "sample": f"N={N} FP64 advective-form oracle, config run steps 1-{steps} (NEXTDT at it=1, "
+                      f"{UPDATE}), median of the {len(secs)} step bodies; {wall:.1f} s in total",
+            "host": facts, "parity": parity,
+            "n29400": {"status": "not run",
+                       "reason": f"the oracle holds about {big_need:.0f} GB of host arrays at N={BIG['N']} "
+                                 f"(MemAvailable {facts.get('MemAvailable_GB')} GB) and takes minutes per step; "
+                                 "the multi-pass path is checked against it at N=64..1024 and on "
+                                 "low-mode states at N=29400 by the GPU tests"}}
 
 
 def big_leg(N: int) -> dict:
@@ -252,10 +305,12 @@ def main() -> int:
             if it == 1 or it % UPDATE == 0:
                 s.next_dt()
 
+    every = max(1, args.steps // TIMING_SAMPLES)
+
     def timed(s):
         advance(s, args.warmup)
         s.synchronize()
-        s.set_kernel_timing(True, every=TIMING_EVERY)
+        s.set_kernel_timing(True, every=every)
         s.reset_kernel_timing()
         s.synchronize()
         t0 = time.perf_counter()
@@ -274,23 +329,37 @@ def main() -> int:
     dev_bytes = s.device_bytes()
     s.close()
 
+    # the parity checkpoint: the config run to step 100 (NEXTDT at it = 1 and 100) in FP64
+    # and in mixed precision, compared with the oracle in the cpu_baseline leg
+    PARITY_STEPS = 100
+    a = make(oa.PREC_FP64)
+    gpu_states = {"om0": None, "om64": None, "om32": None}
+    a0 = oa.Solver(N, WORKLOAD["Re"], WORKLOAD["K0"], WORKLOAD["CFL"])
+    a0.init_ic(oa.IC_K4EXP, WORKLOAD["seed"])
+    gpu_states["om0"] = a0.om2()
+    a0.close()
+    advance(a, PARITY_STEPS)
+    gpu_states["om64"] = a.om2()
+    a.close()
+
     # mixed precision: FP32 transforms, FP64 state / CN-AB2 / CFL-dt / E, Z
     mixed = None
     if not args.fp64_only:
         s32 = make(oa.PREC_FP32)
         wall32, kt32 = timed(s32)
         s32.close()
-        a, b = make(oa.PREC_FP64), make(oa.PREC_FP32)
-        advance(a, 100)
-        advance(b, 100)
-        oa64, oa32 = a.om2(), b.om2()
+        b = make(oa.PREC_FP32)
+        advance(b, PARITY_STEPS)
+        oa32 = gpu_states["om32"] = b.om2()
+        oa64 = gpu_states["om64"]
         err = float(np.linalg.norm(oa32 - oa64) / np.linalg.norm(oa64))
-        a.close()
+        derr = float(np.linalg.norm(oa32 - oa64) / np.linalg.norm(oa64 - gpu_states["om0"]))
         b.close()
         ms32 = 1e3 * wall32 / args.steps
         mixed = {"precision": "fp32 transforms + fp64 state", "ms_per_step": round(ms32, 4),
                  "speedup_vs_fp64": round((1e3 * wall / args.steps) / ms32, 3),
                  "rel_l2_vorticity_err_vs_fp64_step100": err,
+                 "rel_l2_vorticity_err_vs_fp64_step100_per_change": derr,
                  "kernels": {n: round(d["total_ms"] / max(1, d["launches"]), 5) for n, d in kt32.items()
                              if d["launches"]}}
 
@@ -305,7 +374,8 @@ def main() -> int:
     dom = max(kt, key=lambda k: kt[k]["total_ms"])
     dom_avg_ms = kt[dom]["total_ms"] / max(1, kt[dom]["launches"])
     achieved = kt[dom]["bytes_per_launch"] / (dom_avg_ms * 1e-3) / 1e9
-    step_bytes = 152.0 * N * N  # 2.5 c_s + 7.5 c_i with the om2 re-read fused away
+    step_bytes = STEP_BYTES_PER_N2 * N * N
+    step_gbps = step_bytes / (ms_per_step * 1e-3) / 1e9
     out = {
         "metric": "step time at N=4096 & 29400, FP64 vs mixed-prec at stated L2 err; HBM GB/s",
         "value": round(ms_per_step, 4),
@@ -324,9 +394,12 @@ def main() -> int:
                                f"K0=4, CFL=0.1, NEXTDT every {UPDATE} steps",
                    "N": N, "padded": 3 * N // 2, "Re": WORKLOAD["Re"], "K0": WORKLOAD["K0"],
                    "CFL": WORKLOAD["CFL"], "parallelism": "single GPU"},
-        "step_hbm_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "step_hbm_GBps": round(step_gbps, 1),
+        "step_roofline": {"bound": "hbm", "bytes_per_step": step_bytes, "achieved": round(step_gbps, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbps / HBM_PEAK_GBS, 4),
+                          "floor_ms": round(step_bytes / (HBM_PEAK_GBS * 1e9) * 1e3, 4)},
         "kernels": per_kernel,
-        "kernel_timing": f"HIP events on every {TIMING_EVERY}th launch of each kernel in the timed region",
+        "kernel_timing": f"HIP events on every {every}th launch of each kernel in the timed region",
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": _traffic_from_profile(dom, N)},
@@ -349,7 +422,7 @@ def main() -> int:
                                                          "enstrophy_rel_err_max", "final_energy_ratio")}
                                    for k, v in c["runs"].items()}}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(N)
+        out["cpu_baseline"] = cpu_baseline(N, PARITY_STEPS, gpu_states)
     print(json.dumps(_finite(out)), flush=True)
     return 0 if finite else 1
 

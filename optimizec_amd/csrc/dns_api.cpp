@@ -49,10 +49,16 @@ enum {
     KT_BIG_XFWD_B,
     KT_BIG_ZFWD_A,
     KT_BIG_ZFWD_B,
+    // passes 1-3 for the gradient pair only, on the step after a NEXTDT (half-size launches)
+    KT_BIG_ZINV_A_HALF,
+    KT_BIG_ZINV_B_HALF,
+    KT_BIG_XINV_A_HALF,
     KT_COUNT
 };
-const char *const kKernelNames[KT_COUNT] = {"k_row",      "k_col",      "k_row_cfl",  "big_zinv_a", "big_zinv_b",
-                                            "big_xinv_a", "big_xinv_b", "big_xfwd_b", "big_zfwd_a", "big_zfwd_b"};
+const char *const kKernelNames[KT_COUNT] = {"k_row",           "k_col",           "k_row_cfl",      "big_zinv_a",
+                                            "big_zinv_b",      "big_xinv_a",      "big_xinv_b",     "big_xfwd_b",
+                                            "big_zfwd_a",      "big_zfwd_b",      "big_zinv_a_half", "big_zinv_b_half",
+                                            "big_xinv_a_half"};
 
 }  // namespace
 
@@ -255,7 +261,8 @@ dns_status big_step(dns_handle *h, double dt)
         // plane 0 of pass 3 (u + i w) is in buffer A from the NEXTDT on this om (same
         // kernels, same arithmetic): passes 1-3 for the gradient pair only, its z-pass-A
         // output in buffer A past plane 0, its rows at their place (fields 2, 3) in buffer B,
-        // its plane 1 of pass 3 at its place in buffer A.  Untimed: half-size launches.
+        // its plane 1 of pass 3 at its place in buffer A.  Timed under their own names
+        // (half-size launches).
         char *ba = static_cast<char *>(h->BA), *bb = static_cast<char *>(h->BB);
         const size_t e = sizeof(cd), plane = (size_t)h->M * h->ms * e, field = (size_t)h->M * h->hs * e;
         BigReq z{};
@@ -267,16 +274,16 @@ dns_status big_step(dns_handle *h, double dt)
         z.fields.pairs = 1;
         z.fields.f[0] = fg[0];
         z.fields.f[1] = fg[1];
-        if ((st = big_launch(h, -1, z)) != DNS_OK) return st;
+        if ((st = big_launch(h, KT_BIG_ZINV_A_HALF, z)) != DNS_OK) return st;
         z.what = BIG_ZINV_B;
         z.a = ba + plane;
         z.b = bb + 2 * field;
-        if ((st = big_launch(h, -1, z)) != DNS_OK) return st;
+        if ((st = big_launch(h, KT_BIG_ZINV_B_HALF, z)) != DNS_OK) return st;
         r.what = BIG_XINV_A;
         r.planes = 1;
         r.a = bb + 2 * field;
         r.b = ba + plane;
-        if ((st = big_launch(h, -1, r)) != DNS_OK) return st;
+        if ((st = big_launch(h, KT_BIG_XINV_A_HALF, r)) != DNS_OK) return st;
     } else {
         if ((st = big_zinv(h, false, f4, 2, true)) != DNS_OK) return st;
         r.what = BIG_XINV_A;
@@ -1234,11 +1241,14 @@ extern "C" dns_status dns_kernel_timing(dns_handle *h, const char **names, int64
     //   xinv_a  4 fields in (3 c), 2 planes out (4.5 c)   xinv_b  2 planes in, 1/2 plane out (5.625 c)
     //   xfwd_b  1/2 plane in, 2/3 of it out (1.875 c)     zfwd_a  kept modes in, 1 field out (1.5 c)
     //   zfwd_b  1 field in (0.75 c), om, fn in and out (32 N^2)
+    //   the half-size passes 1-3 (gradient pair only, FP64): zinv_a 8 N^2 + 1.5 c, zinv_b 3 c,
+    //   xinv_a 2 fields in (1.5 c), 1 plane out (2.25 c)
     const double c = h->fp32 ? 8.0 : 16.0;
     const double bytes[KT_COUNT] = {3.75 * c * N2,        (3.75 * c + 32.0) * N2, 24.0 * N2,
                                     (8.0 + 3.0 * c) * N2, 6.0 * c * N2,           7.5 * c * N2,
                                     5.625 * c * N2,       1.875 * c * N2,         1.5 * c * N2,
-                                    (0.75 * c + 32.0) * N2};
+                                    (0.75 * c + 32.0) * N2, (8.0 + 1.5 * c) * N2, 3.0 * c * N2,
+                                    3.75 * c * N2};
     const int cnt = std::min(*n, (int)KT_COUNT);
     for (int i = 0; i < cnt; ++i) {
         if (names) names[i] = kKernelNames[i];

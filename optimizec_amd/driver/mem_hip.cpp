@@ -620,7 +620,15 @@ int main(int argc, char **argv)
     const char *prec_env = std::getenv("DNS_PREC");
     const int prec = (prec_env && (iequals(prec_env, "fp32") || iequals(prec_env, "mixed"))) ? DNS_PREC_FP32 : DNS_PREC_FP64;
     if (prec == DNS_PREC_FP32) std::printf(" precision: FP32 transforms, FP64 state (DNS_PREC=%s)\n", prec_env);
-    dns_config cfg{N, Re, K0, CFL, prec, 0, DNS_PATH_AUTO};
+    // DNS_PATH=fused / multipass picks the kernel path where both are built (default: auto)
+    const char *path_env = std::getenv("DNS_PATH");
+    const int path = !path_env ? DNS_PATH_AUTO
+                     : iequals(path_env, "fused") ? DNS_PATH_FUSED
+                     : iequals(path_env, "multipass") ? DNS_PATH_MULTIPASS
+                                                      : DNS_PATH_AUTO;
+    if (path != DNS_PATH_AUTO)
+        std::printf(" path: %s (DNS_PATH)\n", path == DNS_PATH_FUSED ? "fused" : "multi-pass");
+    dns_config cfg{N, Re, K0, CFL, prec, 0, path};
     dns_status st = dns_create(&cfg, &r.h);
     if (st != DNS_OK) {
         std::fprintf(stderr, "dns_create: %s\n", dns_last_error(nullptr));
@@ -710,7 +718,10 @@ int main(int argc, char **argv)
         }
         if (it == 1 || it % UPDATE == 0 || it == STEPS) {
             double sigma = 0.0, pal = 0.0;
-            if (!check(dns_next_dt(r.h), r, "next_dt") || !check(dns_display_sigma(r.h, &sigma), r, "sigma")) {
+            // SIG before NEXTDT: both only read om, so the printed line is the same in either
+            // order, and NEXTDT last leaves the velocity rows of its CFL pass in place for the
+            // next step of the multi-pass path (display sigma reuses those buffers)
+            if (!check(dns_display_sigma(r.h, &sigma), r, "sigma") || !check(dns_next_dt(r.h), r, "next_dt")) {
                 dns_destroy(r.h);
                 return 1;
             }

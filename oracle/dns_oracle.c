@@ -44,6 +44,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -1198,14 +1199,33 @@ void or_one_step(or_state *S, int form)
     }
 }
 
+static double wall_seconds(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+#endif
+}
+
 int or_run(or_state *S, int steps, int update, int form, double *series, int maxrows)
+{
+    return or_run_timed(S, steps, update, form, series, maxrows, NULL);
+}
+
+int or_run_timed(or_state *S, int steps, int update, int form, double *series, int maxrows, double *step_seconds)
 {
     int rows = 0;
     if (update <= 0) update = 100;
+    const int it0 = S->it;
     for (int it = S->it + 1; it <= steps; ++it) {
         S->it = it;
         const double dt_old = S->dt;
+        const double w0 = step_seconds ? wall_seconds() : 0.0;
         or_one_step(S, form);
+        if (step_seconds) step_seconds[it - it0 - 1] = wall_seconds() - w0;
         S->t += dt_old;
         if (it == 1 || (it % update) == 0 || it == steps) {
             if (form != 0) or_velocity_from_om2(S);

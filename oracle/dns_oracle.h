@@ -82,6 +82,10 @@ void or_velocity_from_om2(or_state *S);
  * row) if series != NULL, up to maxrows rows.  Returns number of rows written. */
 int  or_start(or_state *S, int form);   /* initial step2a + next_dt (it=0) */
 int  or_run(or_state *S, int steps, int update, int form, double *series, int maxrows);
+/* or_run, and the wall time of each step's loop body (or_one_step, without the
+ * diagnostics) in step_seconds[0 .. steps - it0 - 1] when it is not NULL. */
+int  or_run_timed(or_state *S, int steps, int update, int form, double *series, int maxrows,
+                  double *step_seconds);
 /* One step of the loop body, without diagnostics. */
 void or_one_step(or_state *S, int form);
 

@@ -71,6 +71,8 @@ def _declare(L):
     L.or_one_step.argtypes = [_P, ctypes.c_int]
     L.or_run.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int]
     L.or_run.restype = ctypes.c_int
+    L.or_run_timed.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _D]
+    L.or_run_timed.restype = ctypes.c_int
     L.or_energy_enstrophy.argtypes = [_P, _D, _D]
     L.or_cflm.argtypes = [_P]
     L.or_cflm.restype = ctypes.c_double
@@ -173,6 +175,14 @@ class Oracle:
         buf = np.zeros((maxrows, 7), dtype=np.float64)
         n = lib().or_run(self._h, steps, update, form, _ptr(buf), maxrows)
         return buf[:n].copy()
+
+    def run_timed(self, steps: int, update: int, form: int = 1, maxrows: int = 100000):
+        """run(), plus the wall seconds of each step's loop body (diagnostics excluded)."""
+        buf = np.zeros((maxrows, 7), dtype=np.float64)
+        n_steps = max(0, steps - self.scalars().it)
+        secs = np.zeros(max(1, n_steps), dtype=np.float64)
+        n = lib().or_run_timed(self._h, steps, update, form, _ptr(buf), maxrows, _ptr(secs))
+        return buf[:n].copy(), secs[:n_steps].copy()
 
     def step2a(self): lib().or_step2a(self._h)
     def step2b(self): lib().or_step2b(self._h)

@@ -47,6 +47,25 @@ def _iterations(stdout):
     return rows
 
 
+def test_driver_multipass_lines_match_fused(tmp_path):
+    """The driver on the multi-pass path (DNS_PATH=multipass, N=1024): the step after each
+    diagnostic step reuses the velocity rows NEXTDT leaves in place (display sigma runs
+    before NEXTDT), and the ITERATION lines equal the fused path's to the printed digits."""
+    args = ["1024", "1e4", "4", "60", "0.1", "20", "0"]
+    env = dict(os.environ, DNS_IC="k4exp", DNS_SEED="1")
+    runs = {}
+    for path in ("fused", "multipass"):
+        r = subprocess.run([DRIVER, *args], cwd=tmp_path, env=dict(env, DNS_PATH=path), capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        runs[path] = _iterations(r.stdout)
+    assert sorted(runs["fused"]) == [1, 20, 40, 60]
+    for it, f in runs["fused"].items():
+        m = runs["multipass"][it]
+        for key in ("T", "DT", "CN", "CFLM", "SIG"):
+            assert m[key] == f[key], (it, key, m[key], f[key])
+
+
 def test_driver_mixed_precision_knob(tmp_path):
     """DNS_PREC=fp32 runs the mixed-precision step from the command line: the ITERATION
     lines follow the FP64 run's (T, DT, CN, CFLM to 1e-6 relative; NEXTDT runs in FP64) and

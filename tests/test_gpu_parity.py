@@ -72,6 +72,34 @@ def test_start_and_single_step(oracle_mod, N):
     s.close()
 
 
+def _series_errors(rs, ro):
+    """Largest relative difference of t, dt, cn, cflm, E, Z over the diagnostic rows."""
+    assert rs.shape == ro.shape
+    assert np.array_equal(rs[:, 0], ro[:, 0])
+    return {name: float(np.max(np.abs(rs[:, col] - ro[:, col]) / np.abs(ro[:, col])))
+            for col, name in ((1, "t"), (2, "dt"), (3, "cn"), (4, "cflm"), (5, "E"), (6, "Z"))}
+
+
+def test_run_512_config_1000_steps(oracle_mod):
+    """The N=512 configuration in full: Re=1e4, K0=4, CFL=0.1, seed 0, 1000 steps with
+    diagnostics (NEXTDT, E, Z) every 100 -- rel. L2 vorticity <= 1e-11 at the end and the
+    t/dt/cn/CFLM/E/Z series <= 1e-9 at every diagnostic step."""
+    N, steps, update = 512, 1000, 100
+    s, o = _pair(oracle_mod, N, Re=1e4, K0=4.0, cfl=0.1, seed=0)
+    s.start()
+    o.start(form=1)
+    rs = s.run(steps, update)
+    ro = o.run(steps, update, form=1)
+    assert rs.shape[0] == 2 + steps // update - 1  # it = 1, 100, 200, ..., 1000
+    errs = _series_errors(rs, ro)
+    err = relerr(s.om2(), o.om2())
+    print(f"N=512 {steps} steps: rel L2 vorticity {err:.3e}; series {errs}")
+    for name, e in errs.items():
+        assert e < 1e-9, (name, e)
+    assert err < 1e-11
+    s.close()
+
+
 def test_run_512_100_steps(oracle_mod):
     """Config IC, N=512, Re=1e4, K0=4, CFL=0.1: 100 steps, diagnostics every 20."""
     N, steps, update = 512, 100, 20
@@ -106,6 +134,63 @@ def test_pao_compat_start(oracle_mod):
     o.om2phys()
     assert int(s.display_sigma()) == int(o.display_sigma(2))
     assert abs(s.display_sigma() - o.display_sigma(2)) < 1e-6
+    s.close()
+
+
+@pytest.fixture(scope="module")
+def oracle_4096(oracle_mod):
+    """The FP64 oracle on the N=4096 configuration (Re=1e5, K0=4, CFL=0.1, seed 0): 100
+    steps with diagnostics every 100 (NEXTDT at it = 1 and 100), state at steps 0 and 100."""
+    N, steps, update = 4096, 100, 100
+    o = oracle_mod.Oracle(N, 1e5, 4.0, 0.1)
+    o.init_config(0)
+    om0 = o.om2()
+    o.start(form=1)
+    print(f"oracle N={N}: {steps} steps", flush=True)
+    ro = o.run(steps, update, form=1)
+    return {"om0": om0, "om": o.om2(), "rows": ro, "steps": steps, "update": update}
+
+
+def test_4096_config_100_steps_against_oracle(oracle_4096):
+    """N=4096 (6144^2 padded), Re=1e5, K0=4, CFL=0.1, seed 0: 100 FP64 steps against the
+    oracle -- rel. L2 vorticity <= 1e-11 at step 100, and t/dt/cn/CFLM/E/Z <= 1e-9 at
+    every diagnostic step (it = 1, 100)."""
+    s = oa.Solver(4096, 1e5, 4.0, 0.1)
+    s.init_ic(oa.IC_K4EXP, 0)
+    assert np.array_equal(s.om2(), oracle_4096["om0"])
+    s.start()
+    rs = s.run(oracle_4096["steps"], oracle_4096["update"])
+    errs = _series_errors(rs, oracle_4096["rows"])
+    om = s.om2()
+    err = relerr(om, oracle_4096["om"])
+    # the same error measured against the change of the field over the run
+    derr = float(np.linalg.norm((om - oracle_4096["om"]).ravel())
+                 / np.linalg.norm((oracle_4096["om"] - oracle_4096["om0"]).ravel()))
+    print(f"N=4096 FP64 100 steps: rel L2 vorticity {err:.3e} (vs |w100 - w0|: {derr:.3e}); series {errs}")
+    for name, e in errs.items():
+        assert e < 1e-9, (name, e)
+    assert err < 1e-11
+    s.close()
+
+
+def test_4096_fp32_transforms_against_oracle(oracle_4096):
+    """Mixed precision at N=4096 (FP32 transforms, FP64 state, CN/AB2, CFL/dt, E/Z) against
+    the FP64 oracle after 100 steps: rel. L2 vorticity <= 1e-5.  The error is also reported
+    relative to |w_100 - w_0| (how well the FP32 transforms carry the evolution itself),
+    which must stay below 1e-2."""
+    s = oa.Solver(4096, 1e5, 4.0, 0.1, precision=oa.PREC_FP32)
+    s.init_ic(oa.IC_K4EXP, 0)
+    s.start()
+    rs = s.run(oracle_4096["steps"], oracle_4096["update"])
+    om = s.om2()
+    err = relerr(om, oracle_4096["om"])
+    derr = float(np.linalg.norm((om - oracle_4096["om"]).ravel())
+                 / np.linalg.norm((oracle_4096["om"] - oracle_4096["om0"]).ravel()))
+    errs = _series_errors(rs, oracle_4096["rows"])
+    print(f"N=4096 FP32 transforms 100 steps: rel L2 vorticity {err:.3e} "
+          f"(vs |w100 - w0|: {derr:.3e}); series {errs}")
+    assert 1e-13 < err < 1e-5
+    assert derr < 1e-2
     s.close()
 
 
