@@ -48,6 +48,10 @@ TIMING_SAMPLES = 10
 STEP_BYTES_PER_N2 = 160.0
 
 WORKLOAD = {"N": 4096, "Re": 1.0e5, "K0": 4.0, "CFL": 0.1, "seed": 0}
+# mixed-precision variants run beside FP64 (name, dns_precision): FP32 vector radix stages;
+# the radix stages as DFT-matrix products on the matrix cores with three BF16 products, or
+# one power-of-two-scaled FP16 product (FP32 intermediates, FP64 state/CN/dt/E/Z in all)
+MIXED_VARIANTS = (("fp32", 1), ("bf16x3", 2), ("fp16", 3))
 BIG = {"N": 29400, "Re": 1.0e7, "K0": 4.0, "CFL": 0.1, "seed": 0, "steps": 41, "update": 20}
 
 
@@ -118,7 +122,8 @@ def cpu_baseline(N: int, steps: int, gpu: dict):
     o.close()
     change = float(np.linalg.norm(om - gpu["om0"]))
     parity = {"step": steps}
-    for key, name in (("om64", "gpu_fp64"), ("om32", "gpu_mixed_fp32")):
+    keys = [("om64", "gpu_fp64")] + [("om_" + n, "gpu_mixed_" + n) for n, _ in MIXED_VARIANTS]
+    for key, name in keys:
         if gpu.get(key) is not None:
             d = float(np.linalg.norm(gpu[key] - om))
             parity[f"rel_l2_vorticity_{name}_vs_oracle"] = d / float(np.linalg.norm(om))
@@ -173,25 +178,47 @@ def big_leg(N: int) -> dict:
                 for k, d in kt.items() if d["launches"]}
         return 1e3 * wall / (steps - 1), phases, gbps, E, Z, om, dev
 
+    s0 = oa.Solver(N, BIG["Re"], BIG["K0"], BIG["CFL"], precision=oa.PREC_FP64)
+    s0.init_ic(oa.IC_K4EXP, BIG["seed"])
+    om0 = s0.om2()
+    s0.close()
     ms64, ph64, gb64, E, Z, om64, dev64 = run(oa.PREC_FP64)
-    ms32, ph32, _, E32, Z32, om32, dev32 = run(oa.PREC_FP32)
-    err = float(np.linalg.norm(om32 - om64) / np.linalg.norm(om64))
-    del om32, om64
+    ref = float(np.linalg.norm(om64))
+    change = float(np.linalg.norm(om64 - om0))
+    del om0
+    variants = {}
+    for name, prec in MIXED_VARIANTS:
+        try:
+            ms, ph, _, Ev, Zv, om, dev = run(prec)
+        except oa.DnsError as e:  # not built for this N
+            variants[name] = {"status": f"not built: {e}"}
+            continue
+        d = float(np.linalg.norm(om - om64))
+        del om
+        variants[name] = {"ms_per_step": round(ms, 3), "speedup_vs_fp64": round(ms64 / ms, 3), "phase_ms": ph,
+                          "device_GB": round(dev / 1e9, 1),
+                          "rel_l2_vorticity_err_vs_fp64_step41": d / ref,
+                          "rel_l2_vorticity_err_vs_fp64_step41_per_change": d / change,
+                          "final": {"E": Ev, "Z": Zv}}
+    del om64
+    v32 = variants["fp32"]
     return {
         "workload": f"2D vorticity DNS, N={N} ({3 * N // 2}^2 padded), Re=1e7, K0=4, CFL=0.1, "
                     f"{BIG['steps']} steps, NEXTDT at it=1, {BIG['update']}, 40, 41; steps 2-41 timed",
         "path": "multi-pass (batched four-step line FFTs through HBM)",
         "fp64_ms_per_step": round(ms64, 3),
-        "fp64_step_hbm_GBps": round(160.0 * N * N / (ms64 * 1e-3) / 1e9, 1),
+        "fp64_step_hbm_GBps": round(STEP_BYTES_PER_N2 * N * N / (ms64 * 1e-3) / 1e9, 1),
+        "fp64_step_roofline_frac": round(STEP_BYTES_PER_N2 * N * N / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "fp64_phase_ms": ph64,
         "fp64_phase_GBps": gb64,
         "fp64_device_GB": round(dev64 / 1e9, 1),
-        "mixed_ms_per_step": round(ms32, 3),
-        "mixed_speedup_vs_fp64": round(ms64 / ms32, 3),
-        "mixed_phase_ms": ph32,
-        "mixed_device_GB": round(dev32 / 1e9, 1),
-        "mixed_rel_l2_vorticity_err_vs_fp64_step41": err,
-        "final_fp64": {"E": E, "Z": Z}, "final_mixed": {"E": E32, "Z": Z32},
+        "mixed_ms_per_step": v32["ms_per_step"],
+        "mixed_speedup_vs_fp64": v32["speedup_vs_fp64"],
+        "mixed_phase_ms": v32["phase_ms"],
+        "mixed_device_GB": v32["device_GB"],
+        "mixed_rel_l2_vorticity_err_vs_fp64_step41": v32["rel_l2_vorticity_err_vs_fp64_step41"],
+        "mixed_variants": variants,
+        "final_fp64": {"E": E, "Z": Z},
     }
 
 
@@ -333,7 +360,7 @@ def main() -> int:
     # and in mixed precision, compared with the oracle in the cpu_baseline leg
     PARITY_STEPS = 100
     a = make(oa.PREC_FP64)
-    gpu_states = {"om0": None, "om64": None, "om32": None}
+    gpu_states = {"om0": None, "om64": None}
     a0 = oa.Solver(N, WORKLOAD["Re"], WORKLOAD["K0"], WORKLOAD["CFL"])
     a0.init_ic(oa.IC_K4EXP, WORKLOAD["seed"])
     gpu_states["om0"] = a0.om2()
@@ -342,26 +369,39 @@ def main() -> int:
     gpu_states["om64"] = a.om2()
     a.close()
 
-    # mixed precision: FP32 transforms, FP64 state / CN-AB2 / CFL-dt / E, Z
+    # mixed precision: FP32 transforms, FP64 state / CN-AB2 / CFL-dt / E, Z; the FP32
+    # vector stages and the matrix-core variants, each timed and run to the checkpoint
     mixed = None
     if not args.fp64_only:
-        s32 = make(oa.PREC_FP32)
-        wall32, kt32 = timed(s32)
-        s32.close()
-        b = make(oa.PREC_FP32)
-        advance(b, PARITY_STEPS)
-        oa32 = gpu_states["om32"] = b.om2()
         oa64 = gpu_states["om64"]
-        err = float(np.linalg.norm(oa32 - oa64) / np.linalg.norm(oa64))
-        derr = float(np.linalg.norm(oa32 - oa64) / np.linalg.norm(oa64 - gpu_states["om0"]))
-        b.close()
-        ms32 = 1e3 * wall32 / args.steps
-        mixed = {"precision": "fp32 transforms + fp64 state", "ms_per_step": round(ms32, 4),
-                 "speedup_vs_fp64": round((1e3 * wall / args.steps) / ms32, 3),
-                 "rel_l2_vorticity_err_vs_fp64_step100": err,
-                 "rel_l2_vorticity_err_vs_fp64_step100_per_change": derr,
-                 "kernels": {n: round(d["total_ms"] / max(1, d["launches"]), 5) for n, d in kt32.items()
-                             if d["launches"]}}
+        variants = {}
+        for name, prec in MIXED_VARIANTS:
+            try:
+                sv = make(prec)
+            except oa.DnsError as e:  # not built for this N
+                variants[name] = {"status": f"not built: {e}"}
+                continue
+            wallv, ktv = timed(sv)
+            sv.close()
+            b = make(prec)
+            advance(b, PARITY_STEPS)
+            om = gpu_states["om_" + name] = b.om2()
+            b.close()
+            msv = 1e3 * wallv / args.steps
+            variants[name] = {
+                "ms_per_step": round(msv, 4), "speedup_vs_fp64": round((1e3 * wall / args.steps) / msv, 3),
+                "rel_l2_vorticity_err_vs_fp64_step100": float(np.linalg.norm(om - oa64) / np.linalg.norm(oa64)),
+                "rel_l2_vorticity_err_vs_fp64_step100_per_change":
+                    float(np.linalg.norm(om - oa64) / np.linalg.norm(oa64 - gpu_states["om0"])),
+                "kernels": {n: round(d["total_ms"] / max(1, d["launches"]), 5) for n, d in ktv.items()
+                            if d["launches"]}}
+        v32 = variants["fp32"]
+        mixed = {"precision": "fp32 transforms + fp64 state", "ms_per_step": v32["ms_per_step"],
+                 "speedup_vs_fp64": v32["speedup_vs_fp64"],
+                 "rel_l2_vorticity_err_vs_fp64_step100": v32["rel_l2_vorticity_err_vs_fp64_step100"],
+                 "rel_l2_vorticity_err_vs_fp64_step100_per_change":
+                     v32["rel_l2_vorticity_err_vs_fp64_step100_per_change"],
+                 "kernels": v32["kernels"], "variants": variants}
 
     ms_per_step = 1e3 * wall / args.steps
     per_kernel = {}

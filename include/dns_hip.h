@@ -46,9 +46,15 @@ typedef enum {
 
 typedef enum {
     DNS_PREC_FP64 = 0,  /* everything in FP64                                              */
-    DNS_PREC_FP32 = 1   /* transforms and their intermediates in FP32; state om2/fnm1, the
+    DNS_PREC_FP32 = 1,  /* transforms and their intermediates in FP32; state om2/fnm1, the
                            CN/AB2 update, CFL/dt, energy/enstrophy and twiddle generation
                            in FP64 (N >= 128)                                                */
+    DNS_PREC_BF16X3 = 2,/* as DNS_PREC_FP32, with every radix stage (radix <= 16) of the line
+                           transforms run as a DFT-matrix product on the matrix cores: three
+                           BF16 products (hi*hi + hi*lo + lo*hi), FP32 accumulation
+                           (N = 256, 512, 1024, 2048, 4096; N = 29400 on the multi-pass path) */
+    DNS_PREC_FP16 = 3   /* as DNS_PREC_BF16X3 with one FP16 product per stage, each butterfly's
+                           inputs scaled by a power of two into (-1, 1) (same sizes)          */
 } dns_precision;
 
 typedef enum {

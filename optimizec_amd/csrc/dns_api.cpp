@@ -78,6 +78,7 @@ struct dns_handle {
     int N = 0, M = 0, H = 0, NK = 0;
     int device = 0;
     int fp32 = 0;               // 1: V, G and the transforms in FP32 (state stays FP64)
+    int engine = 0;             // with fp32: line-transform stages 0 vector, 1 BF16x3 / 2 FP16 matrix cores
     bool big = false;           // multi-pass path (dns_big.hip) instead of the fused kernels
     void *BA = nullptr;         // multi-pass work buffers (dns_big.h layouts): Y, X, Qh and the
     void *BB = nullptr;         // diagnostics rows in BA; Z, Yq, W, diagnostics plane in BB
@@ -233,6 +234,7 @@ dns_status big_zinv(dns_handle *h, bool fp64, const int *fields, int pairs, bool
     BigReq r{};
     r.what = BIG_ZINV_A;
     r.fp32 = fp64 ? 0 : h->fp32;
+    r.engine = r.fp32 ? h->engine : 0;
     r.tw = r.fp32 ? h->twT : h->tw;
     r.om = h->om;
     r.b = h->BA;
@@ -256,6 +258,7 @@ dns_status big_step(dns_handle *h, double dt)
     dns_status st;
     BigReq r{};
     r.fp32 = h->fp32;
+    r.engine = h->engine;
     r.tw = h->fp32 ? h->twT : h->tw;
     if (reuse) {
         // plane 0 of pass 3 (u + i w) is in buffer A from the NEXTDT on this om (same
@@ -416,6 +419,7 @@ dns_status refresh_v(dns_handle *h)
     LaunchReq r{};
     r.what = LAUNCH_COL;
     r.fp32 = h->fp32;
+    r.engine = h->engine;
     r.om = h->om;
     r.fn = h->fn;
     r.G = h->G;
@@ -644,10 +648,17 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     if (!dns_supported_path(cfg->N, cfg->path))
         return fail(nullptr, DNS_E_UNSUPPORTED, "N=%d: no kernels built for H=N/2=%d, M=3N/2=%d on path %d", cfg->N,
                     cfg->N / 2, 3 * cfg->N / 2, cfg->path);
-    if (cfg->precision != DNS_PREC_FP64 && cfg->precision != DNS_PREC_FP32)
+    if (cfg->precision < DNS_PREC_FP64 || cfg->precision > DNS_PREC_FP16)
         return fail(nullptr, DNS_E_UNSUPPORTED, "precision %d is not built", cfg->precision);
-    if (cfg->precision == DNS_PREC_FP32 && cfg->N < 128)
+    if (cfg->precision != DNS_PREC_FP64 && cfg->N < 128)
         return fail(nullptr, DNS_E_UNSUPPORTED, "FP32 transforms are built for N >= 128");
+    if (cfg->precision == DNS_PREC_BF16X3 || cfg->precision == DNS_PREC_FP16) {
+        const bool fused = cfg->path != DNS_PATH_MULTIPASS && dns_kernels_supported(cfg->N / 2);
+        const bool ok = fused ? dns_kernels_mfma_supported(cfg->N / 2) != 0 : dns_big_mfma_supported_m(3 * cfg->N / 2) != 0;
+        if (!ok)
+            return fail(nullptr, DNS_E_UNSUPPORTED, "matrix-core line transforms are not built for N=%d on path %d",
+                        cfg->N, cfg->path);
+    }
     if (!(cfg->Re > 0.0)) return fail(nullptr, DNS_E_ARG, "Re must be positive");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
@@ -661,7 +672,8 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     h->M = 3 * cfg->N / 2;
     h->NK = 3 * cfg->N / 4 + 1;
     h->device = cfg->device;
-    h->fp32 = cfg->precision == DNS_PREC_FP32 ? 1 : 0;
+    h->fp32 = cfg->precision != DNS_PREC_FP64 ? 1 : 0;
+    h->engine = cfg->precision == DNS_PREC_BF16X3 ? 1 : cfg->precision == DNS_PREC_FP16 ? 2 : 0;
     h->big = cfg->path == DNS_PATH_MULTIPASS || (cfg->path == DNS_PATH_AUTO && !dns_kernels_supported(h->H));
     h->sc.N = h->N;
     h->sc.M = h->M;
@@ -826,6 +838,7 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
         LaunchReq r{};
         r.what = LAUNCH_ROW;
         r.fp32 = h->fp32;
+        r.engine = h->engine;
         r.V = h->V;
         r.vstride = vs;
         r.Gw = h->G;
@@ -838,6 +851,7 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
         LaunchReq c{};
         c.what = LAUNCH_COL;
         c.fp32 = h->fp32;
+        c.engine = h->engine;
         c.om = h->om;
         c.fn = h->fn;
         c.G = h->G;

@@ -46,6 +46,7 @@ struct BigCnArgs {
 struct BigReq {
     int what = 0;
     int fp32 = 0;   // element type of the intermediates and tw: cf (1) or cd (0)
+    int engine = 0; // with fp32: line-transform stages 0 vector (cf), 1 BF16x3 (cm3), 2 FP16 (cmh)
     int H = 0;
     hipStream_t stream = nullptr;
     const void *tw = nullptr;  // W_M^t, t in [0, M)
@@ -66,6 +67,8 @@ struct BigReq {
 
 // 1 if line length M = 3N/2 has a built split
 int dns_big_supported_m(int M);
+// 1 if the matrix-core line transforms (engine 1, 2) are built for line length M
+int dns_big_mfma_supported_m(int M);
 // CFL maxima per row pair written by BIG_XINV_CFL
 int dns_big_cfl_tiles(int M);
 hipError_t dns_big_launch(const BigReq &r);

@@ -796,6 +796,18 @@ hipError_t launch_t(const BigReq &r, int M)
     }
 }
 
+// the matrix-core line transforms (cm3, cmh) on the N=29400 grid and on one size the
+// oracle follows (N = 1024)
+template <class TC>
+hipError_t launch_t_mfma(const BigReq &r, int M)
+{
+    switch (M) {
+        case 1536: return launch_m<TC, 1536>(r);
+        case 44100: return launch_m<TC, 44100>(r);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_lines(const BigReq &r, int M)
 {
     switch (M) {
@@ -826,6 +838,8 @@ int dns_big_supported_m(int M)
             return 0;
     }
 }
+
+int dns_big_mfma_supported_m(int M) { return (M == 1536 || M == 44100) ? 1 : 0; }
 
 int dns_big_cfl_tiles(int M)
 {
@@ -860,6 +874,9 @@ hipError_t dns_big_launch(const BigReq &r)
             hipLaunchKernelGGL(k_energy_rows, dim3(2 * r.H), dim3(256), 0, r.stream, r.om, 2 * r.H, r.out0, r.out1);
             return hipGetLastError();
         default:
-            return r.fp32 ? launch_t<cf>(r, M) : launch_t<cd>(r, M);
+            if (!r.fp32) return launch_t<cd>(r, M);
+            if (r.engine == 1) return launch_t_mfma<cm3>(r, M);
+            if (r.engine == 2) return launch_t_mfma<cmh>(r, M);
+            return launch_t<cf>(r, M);
     }
 }

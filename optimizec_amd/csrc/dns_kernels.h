@@ -45,6 +45,7 @@ struct LaunchReq {
     int what;
     hipStream_t stream;
     int fp32;                  // 1: transforms and V, G in FP32 (cf); 0: FP64 (cd)
+    int engine;                // with fp32: 0 vector stages (cf), 1 BF16x3 (cm3), 2 FP16 (cmh) matrix-core stages
     // column kernel
     dnsfft::cd *om, *fn;
     const void *G;             // cd or cf
@@ -65,6 +66,8 @@ struct LaunchReq {
 };
 
 int dns_kernels_supported(int H);
+// 1 if the matrix-core line transforms (engine 1, 2) are built for H
+int dns_kernels_mfma_supported(int H);
 // number of launch configurations of the row / column kernels built for H (variant
 // 0 .. n-1 of LaunchReq); all give bitwise identical results
 int dns_kernels_variants(int H, int fp32);

@@ -980,10 +980,19 @@ hipError_t launch_cfg(const LaunchReq &r)
     return hipGetLastError();
 }
 
+// the matrix-core line transforms (cm3, cmh) are built for the N=4096 grid and the sizes
+// their oracle tests use
+constexpr bool mfma_built(int H) { return H == 128 || H == 256 || H == 512 || H == 1024 || H == 2048; }
+
 template <int H>
 hipError_t launch_all(const LaunchReq &r)
 {
     if (r.fp32) {
+        if constexpr (mfma_built(H)) {
+            if (r.engine == 1) return launch_prec<cm3, H>(r);
+            if (r.engine == 2) return launch_prec<cmh, H>(r);
+        }
+        if (r.engine != 0) return hipErrorInvalidValue;
         if constexpr (H >= 64) return launch_prec<cf, H>(r);
         return hipErrorInvalidValue;
     }
@@ -996,6 +1005,8 @@ int dns_kernels_variants(int H, int fp32)
 {
     return (!fp32 && has_variants(H) && dns_kernels_supported(H)) ? 4 : 1;
 }
+
+int dns_kernels_mfma_supported(int H) { return mfma_built(H) ? 1 : 0; }
 
 int dns_kernels_supported(int H)
 {

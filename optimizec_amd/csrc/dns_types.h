@@ -13,4 +13,15 @@ struct cf {
     float x, y;
 };
 
+// FP32 complex values (8 bytes, as cf) whose line transforms run every radix stage of
+// radix <= 16 as a batched DFT-matrix product on the matrix cores with FP32 accumulation:
+//   cm3 -- three BF16 products per stage (F_hi x_hi + F_hi x_lo + F_lo x_hi)
+//   cmh -- one FP16 product per stage, each butterfly's inputs scaled by a power of two
+struct cm3 {
+    float x, y;
+};
+struct cmh {
+    float x, y;
+};
+
 }  // namespace dnsfft

@@ -52,6 +52,8 @@ __device__ __forceinline__ C muli(C a)
 }
 __device__ __forceinline__ cf to_f(cd a) { return cf{float(a.x), float(a.y)}; }
 __device__ __forceinline__ cd to_d(cf a) { return cd{double(a.x), double(a.y)}; }
+__device__ __forceinline__ cd to_d(cm3 a) { return cd{double(a.x), double(a.y)}; }
+__device__ __forceinline__ cd to_d(cmh a) { return cd{double(a.x), double(a.y)}; }
 __device__ __forceinline__ cd to_d(cd a) { return a; }
 template <class C>
 __device__ __forceinline__ C from_d(cd a);
@@ -59,6 +61,24 @@ template <>
 __device__ __forceinline__ cd from_d<cd>(cd a) { return a; }
 template <>
 __device__ __forceinline__ cf from_d<cf>(cd a) { return to_f(a); }
+template <>
+__device__ __forceinline__ cm3 from_d<cm3>(cd a) { return cm3{float(a.x), float(a.y)}; }
+template <>
+__device__ __forceinline__ cmh from_d<cmh>(cd a) { return cmh{float(a.x), float(a.y)}; }
+
+// 0: vector radix stages; 1: three-BF16-product DFT-matrix stages (cm3); 2: FP16 (cmh)
+template <class C>
+struct mfma_kind {
+    static constexpr int value = 0;
+};
+template <>
+struct mfma_kind<cm3> {
+    static constexpr int value = 1;
+};
+template <>
+struct mfma_kind<cmh> {
+    static constexpr int value = 2;
+};
 
 // Returns x through an empty asm statement: table loads indexed by the result cannot
 // be hoisted out of loops by the compiler (twiddles are loop-invariant in many callers,
@@ -372,16 +392,181 @@ __device__ __forceinline__ void stage(C *buf, const C *__restrict__ tw, int twmu
     __syncthreads();
 }
 
+// ---- one Stockham stage as a DFT-matrix product on the matrix cores ----------------
+// The radix-R DFTs (R <= 16) of 16 butterflies at a time are one product Y = F X on
+// v_mfma_f32_16x16x32_{bf16,f16} (one wavefront; FP32 accumulation), in real form:
+//   columns n (16)  : butterflies
+//   rows m (16)     : output y = 8 t + m / 2 of row tile t (R > 8: t = 0, 1), Re (m even)
+//                     or Im (m odd)
+//   k (32)          : input x = 4 (k / 8) + (k mod 8) / 2, Re (k even) or Im (k odd)
+// A[m][k] is Re/Im of W_R^{+-x y}; lane l supplies A row l % 16 and B column l % 16 at
+// k = 8 (l / 16) .. + 7, and receives D rows 4 (l / 16) .. + 3 of column l % 16, i.e. the
+// complex outputs y = 8 t + 2 (l / 16) and y + 1 of its butterfly (operand layout as
+// documented for the 16 x 16 x 32 instruction; the transforms are checked against the
+// FP64 oracle).  The stage's twiddles multiply the inputs in FP32 before the product.
+//   cm3: A = A_hi + A_lo and X = X_hi + X_lo in BF16 (8-bit mantissas, 16 together);
+//        Y = A_hi X_hi + A_hi X_lo + A_lo X_hi (the A_lo X_lo term, ~2^-18 relative, is
+//        dropped).  R <= 8 packs X_hi (k < 16) and X_lo (k >= 16) into one operand and
+//        takes [A_hi | A_hi] and [A_lo | 0] against it: two products instead of three.
+//   cmh: each butterfly's inputs are scaled by 2^-e (e: exponent of their largest
+//        component, shared by the four lanes of the column) so they lie in (-1, 1), then
+//        rounded to FP16 (11-bit significands); one product; the outputs are scaled by 2^e.
+typedef __bf16 mf_bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 mf_f16x8 __attribute__((ext_vector_type(8)));
+typedef float mf_f32x4 __attribute__((ext_vector_type(4)));
+
+template <int L, int LS, int NL, int T, int R, int NS, bool INV, bool OPQ, class C>
+__device__ __forceinline__ void stage_mfma(C *buf, const C *__restrict__ tw, int twmul)
+{
+    constexpr int KIND = mfma_kind<C>::value;
+    static_assert(KIND == 1 || KIND == 2, "matrix-core stage for cm3 / cmh lines");
+    static_assert(R >= 2 && R <= 16 && T % 64 == 0, "radix <= 16, whole wavefronts");
+    constexpr int LR = L / R;
+    constexpr int NB = NL * LR;              // butterflies
+    constexpr int NT = (NB + 15) / 16;       // tiles of 16 butterflies
+    constexpr int NW = T / 64;
+    constexpr int PT = (NT + NW - 1) / NW;   // tiles per wavefront
+    constexpr int TWS = L / (NS * R);
+    constexpr bool TWO = R > 8;              // two row tiles
+    const int tid = OPQ ? opaque((int)threadIdx.x) : (int)threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int n = lane & 15, q = lane >> 4;
+    const int x0 = TWO ? 4 * q : 4 * (q & 1);  // first of this lane's 4 inputs
+
+    // DFT-matrix operands (per lane, per stage): W_R^e = W_L^{e L / R} from the table
+    mf_bf16x8 ah[2], al[2];
+    mf_f16x8 af[2];
+#pragma unroll
+    for (int t = 0; t < (TWO ? 2 : 1); ++t) {
+        const int y = 8 * t + (n >> 1), c = n & 1;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int x = x0 + (i >> 1), d = i & 1;
+            float coef = 0.0f;
+            if (x < R && y < R) {
+                const C w = tw[opaque(((x * y) % R) * (L / R) * twmul)];
+                const float wi = INV ? -w.y : w.y;
+                coef = (c == 0) ? (d == 0 ? w.x : -wi) : (d == 0 ? wi : w.x);
+            }
+            if constexpr (KIND == 1) {
+                const __bf16 h = (__bf16)coef;
+                ah[t][i] = h;
+                // R <= 8: lanes q >= 2 meet X_lo, which only A_hi multiplies
+                al[t][i] = (!TWO && q >= 2) ? (__bf16)0.0f : (__bf16)(coef - (float)h);
+            } else {
+                af[t][i] = (_Float16)coef;
+            }
+        }
+    }
+
+    // inputs of this lane's butterflies, twiddled, as matrix operands
+    mf_bf16x8 bh[PT], bl[PT];
+    mf_f16x8 bf[PT];
+    int ex[PT];
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        const int tile = wv + p * NW;
+        const int b = tile * 16 + n;
+        float v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = 0.0f;
+        if (tile < NT && b < NB) {
+            const int line = b / LR;
+            const int j = b - line * LR;
+            const int k = j % NS;
+            const C *src = buf + line * LS;
+#pragma unroll
+            for (int i2 = 0; i2 < 4; ++i2) {
+                const int r = x0 + i2;
+                if (r < R) {
+                    C xv = src[pidx(j + r * LR)];
+                    if (NS > 1 && r > 0) {
+                        const C w = tw[opaque(r * k * TWS * twmul)];
+                        xv = INV ? mulc(xv, w) : mul(xv, w);
+                    }
+                    v[2 * i2] = xv.x;
+                    v[2 * i2 + 1] = xv.y;
+                }
+            }
+        }
+        if constexpr (KIND == 1) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const __bf16 h = (__bf16)v[i];
+                const __bf16 l = (__bf16)(v[i] - (float)h);
+                if constexpr (TWO) {
+                    bh[p][i] = h;
+                    bl[p][i] = l;
+                } else {
+                    bh[p][i] = q < 2 ? h : l;  // [X_hi | X_lo] in one operand
+                }
+            }
+        } else {
+            float m = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) m = fmaxf(m, fabsf(v[i]));
+            m = fmaxf(m, __shfl_xor(m, 16, 64));
+            m = fmaxf(m, __shfl_xor(m, 32, 64));
+            int e = 0;
+            (void)frexpf(m, &e);  // m < 2^e (e = 0 for m = 0)
+            ex[p] = e;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) bf[p][i] = (!TWO && q >= 2) ? (_Float16)0.0f : (_Float16)ldexpf(v[i], -e);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        const int tile = wv + p * NW;
+        if (tile < NT) {  // uniform over the wavefront
+            mf_f32x4 acc[2];
+#pragma unroll
+            for (int t = 0; t < (TWO ? 2 : 1); ++t) {
+                acc[t] = mf_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                if constexpr (KIND == 1) {
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], bh[p], acc[t], 0, 0, 0);
+                    if constexpr (TWO) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], bl[p], acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t], bh[p], acc[t], 0, 0, 0);
+                } else {
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(af[t], bf[p], acc[t], 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[t][i] = ldexpf(acc[t][i], ex[p]);
+                }
+            }
+            const int b = tile * 16 + n;
+            if (b < NB) {
+                const int line = b / LR;
+                const int j = b - line * LR;
+                const int k = j % NS;
+                C *dst = buf + line * LS;
+                const int base = (j / NS) * NS * R + k;
+#pragma unroll
+                for (int t = 0; t < (TWO ? 2 : 1); ++t) {
+                    const int y0 = 8 * t + 2 * q;
+                    if (y0 < R) dst[pidx(base + y0 * NS)] = C{acc[t][0], acc[t][1]};
+                    if (y0 + 1 < R) dst[pidx(base + (y0 + 1) * NS)] = C{acc[t][2], acc[t][3]};
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
 template <int L, int LS, int NL, int T, bool INV, int S, int MAXR, bool OPQ, class C>
 __device__ __forceinline__ void stages_from(C *buf, const C *__restrict__ tw, int twmul)
 {
     if constexpr (S < num_stages(L, MAXR)) {
-        stage<L, LS, NL, T, stage_radix(L, S, MAXR), stage_span(L, S, MAXR), INV, OPQ, C>(buf, tw, twmul);
+        constexpr int R = stage_radix(L, S, MAXR);
+        if constexpr (mfma_kind<C>::value != 0 && R <= 16)
+            stage_mfma<L, LS, NL, T, R, stage_span(L, S, MAXR), INV, OPQ, C>(buf, tw, twmul);
+        else
+            stage<L, LS, NL, T, R, stage_span(L, S, MAXR), INV, OPQ, C>(buf, tw, twmul);
         stages_from<L, LS, NL, T, INV, S + 1, MAXR, OPQ, C>(buf, tw, twmul);
     }
 }
 
-// Transform NL lines of length L in place (LDS).  Element i of line l lives at
+// Transform NL lines of length L in place (LDS).  With cm3 / cmh lines every stage of
+// radix <= 16 runs on the matrix cores (stage_mfma) and the plan is the radix-16 one.  Element i of line l lives at
 // buf[l * LS + pidx(i)]; LS >= padded_len(L).  Callers barrier before (data
 // written) -- the first stage reads after the caller's __syncthreads -- and the
 // engine ends with a barrier, so results are visible to every thread on return.
@@ -393,7 +578,9 @@ __device__ __forceinline__ void fft_lines(C *buf, const C *__restrict__ tw, int 
     static_assert(smooth7(L), "line length must factor into 2, 3, 5, 7");
     static_assert(LS >= padded_len(L), "line stride must hold the padded line");
     static_assert(MAXR == 8 || MAXR == 16, "radix plans: 8 or 16");
-    stages_from<L, LS, NL, T, INV, 0, MAXR, OPQ, C>(buf, tw, twmul);
+    // matrix-core lines take the radix-16 plan: fewer, larger products
+    constexpr int MR = mfma_kind<C>::value != 0 ? 16 : MAXR;
+    stages_from<L, LS, NL, T, INV, 0, MR, OPQ, C>(buf, tw, twmul);
 }
 
 }  // namespace dnsfft

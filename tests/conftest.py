@@ -26,3 +26,17 @@ def oracle_mod():
 
     oracle.lib()
     return oracle
+
+
+@pytest.fixture(scope="session")
+def oracle_4096(oracle_mod):
+    """The FP64 oracle on the N=4096 configuration (Re=1e5, K0=4, CFL=0.1, seed 0): 100
+    steps with diagnostics every 100 (NEXTDT at it = 1 and 100), state at steps 0 and 100."""
+    N, steps, update = 4096, 100, 100
+    o = oracle_mod.Oracle(N, 1e5, 4.0, 0.1)
+    o.init_config(0)
+    om0 = o.om2()
+    o.start(form=1)
+    print(f"oracle N={N}: {steps} steps", flush=True)
+    ro = o.run(steps, update, form=1)
+    return {"om0": om0, "om": o.om2(), "rows": ro, "steps": steps, "update": update}

@@ -137,20 +137,6 @@ def test_pao_compat_start(oracle_mod):
     s.close()
 
 
-@pytest.fixture(scope="module")
-def oracle_4096(oracle_mod):
-    """The FP64 oracle on the N=4096 configuration (Re=1e5, K0=4, CFL=0.1, seed 0): 100
-    steps with diagnostics every 100 (NEXTDT at it = 1 and 100), state at steps 0 and 100."""
-    N, steps, update = 4096, 100, 100
-    o = oracle_mod.Oracle(N, 1e5, 4.0, 0.1)
-    o.init_config(0)
-    om0 = o.om2()
-    o.start(form=1)
-    print(f"oracle N={N}: {steps} steps", flush=True)
-    ro = o.run(steps, update, form=1)
-    return {"om0": om0, "om": o.om2(), "rows": ro, "steps": steps, "update": update}
-
-
 def test_4096_config_100_steps_against_oracle(oracle_4096):
     """N=4096 (6144^2 padded), Re=1e5, K0=4, CFL=0.1, seed 0: 100 FP64 steps against the
     oracle -- rel. L2 vorticity <= 1e-11 at step 100, and t/dt/cn/CFLM/E/Z <= 1e-9 at
