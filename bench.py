@@ -17,10 +17,13 @@ The ensemble leg ("ensemble") advances 192 runs at N=256 and 64 runs at N=512 (s
 steps with training pairs every 20 steps, in HIP-graph-replayed blocks, and reports
 member-steps per second.
 
-The closure leg ("closure") trains the subgrid network on the pairs of 64 runs at N=512
-and compares a coarse N=64 run with and without it against the 8x-filtered DNS at N=512
-(Re=1e5, K0=8, t=0.6); the full N=1024 vs N=8192 comparison is run by
-optimizec_amd/closure_eval.py.
+The closure legs train the subgrid network and compare coarse runs with and without it
+against an 8x-filtered DNS: "closure" a coarse N=64 run vs the DNS at N=512 (Re=1e5, K0=8,
+t=0.6, network trained on 64 runs at N=512), where the plain coarse run is badly
+under-resolved; "closure_config" the closure configuration, coarse N=1024 (Re=1e6, K0=8,
+5000 steps) vs the DNS at N=8192, network trained on the 192 x N=256 + 64 x N=512
+ensemble, with the DNS reference read from tests/golden (written once by
+tests/golden/make_closure_reference.py).
 
 Prints one JSON line (see the keys below).  ``--gpus`` above 1 is refused: the solver
 runs one state per GPU, on one GPU.
@@ -241,7 +244,7 @@ def ensemble_leg(steps: int = 2000) -> dict:
     import optimizec_amd as oa
 
     members = ensemble_members()
-    e = oa.Ensemble(members, pair_every=20, filter_ratio=8, max_pairs=steps // 20, autotune=True)
+    e = oa.Ensemble(members, pair_every=20, filter_ratio=8, max_pairs=steps // 20, autotune=2)
     e.init()
     e.synchronize()
     tuning = {N: e.tuning(next(i for i, x in enumerate(members) if x[0] == N)) for N in (256, 512)}
@@ -280,7 +283,7 @@ def ensemble_leg(steps: int = 2000) -> dict:
             "step_hbm_GBps": round(algo / wall / 1e9, 1),
             "pairs_per_member": count, "coarse_n": [n0, 64], "device_GB": round(dev / 1e9, 2),
             "one_by_one_estimate_s": round(seq, 2), "speedup_vs_one_by_one": round(seq / wall, 2),
-            "autotune": {f"N{N}": {"configuration": v, "ms_row_plus_column_inverse": [round(x, 4) for x in ms]}
+            "autotune": {f"N{N}": {"configuration": v, "ms_per_step_by_configuration": [round(x, 4) for x in ms]}
                          for N, (v, ms) in tuning.items()},
             "member0_final": {"it": s0.it, "t": s0.t, "dt": s0.dt}}
 
@@ -461,6 +464,19 @@ def main() -> int:
                           "runs": {k: {q: v[q] for q in ("seconds", "spectrum_rel_err_rms", "spectrum_rel_err_max",
                                                          "enstrophy_rel_err_max", "final_energy_ratio")}
                                    for k, v in c["runs"].items()}}
+        # the closure configuration: coarse N=1024 vs the 8x-filtered FP64 DNS at N=8192, whose
+        # reference (shell spectrum, enstrophy series, GPU seconds) is the committed fixture
+        c5 = closure_eval.evaluate(coarse=1024, ratio=8, Re=1e6, K0=8.0, seed=0, coarse_steps=5000,
+                                   ensemble="config", train_iters=4000,
+                                   dns_cache=os.path.join(ROOT, "tests", "golden",
+                                                          "closure_dns_N8192_Re1e6_K08_seed0.npz"))
+        keys = ("seconds", "spectrum_rel_err_rms", "spectrum_rel_err_max", "spectrum_rel_err_max_resolved",
+                "spectrum_rel_err_max_tail", "enstrophy_rel_err_max", "final_energy_ratio")
+        out["closure_config"] = {"workload": c5["workload"], "ensemble": c5["ensemble"], "training": c5["training"],
+                                 "dns_seconds": c5["dns_seconds"], "dns_from_fixture": c5["dns_from_cache"],
+                                 "resolved_shells": c5["resolved_shells"],
+                                 "gpu_time_ratio_dns_over_closure_run": c5["gpu_time_ratio_dns_over_closure_run"],
+                                 "runs": {k: {q: v[q] for q in keys} for k, v in c5["runs"].items()}}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, PARITY_STEPS, gpu_states)
     print(json.dumps(_finite(out)), flush=True)

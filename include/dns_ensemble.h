@@ -56,10 +56,14 @@ typedef struct {
     int                   max_pairs;     /* training pairs kept per member (device memory)        */
     int                   use_graph;     /* 1: capture a block of pair_every steps in a HIP graph */
     int                   device;
-    int                   autotune;      /* 1: dns_ens_init times the built launch configurations
-                                            of each size and keeps the fastest; 2 + v: every
-                                            size runs configuration v (0 <= v < 4 where the size
-                                            has four, else 0), untimed; 0: configuration 0    */
+    int                   autotune;      /* 1: dns_ens_init times the full step (row pass +
+                                            column kernel) of each launch configuration whose
+                                            results are bitwise the default's and keeps the
+                                            fastest; 2: also the radix-16-plan configurations
+                                            (results then agree with single runs to roundoff);
+                                            3 + v: every size runs configuration v (0 <= v < 9
+                                            where the size has nine, else 0), untimed; 0:
+                                            configuration 0                                     */
 } dns_ens_config;
 
 /* Allocates every member's state and work arrays on the device; no IC yet. */
@@ -86,8 +90,10 @@ dns_status dns_ens_pair_info(dns_ens *e, int m, int *count, int *n);
 /* Pairs first .. first + count - 1 of member m into host: count x 4 x n x n floats. */
 dns_status dns_ens_get_pairs(dns_ens *e, int m, int first, int count, float *host);
 /* Launch configuration chosen for member m's size (0 = default) and the time of one
- * batched row + column-inverse pass of each built configuration (ms[i], i < *n; zeros
- * without autotune).  Every configuration gives bitwise identical results. */
+ * batched step of each built configuration (ms[i], i < *n; 0 where not timed).  The
+ * configurations (column tile x column threads, row threads, opaque FFT index, rows per
+ * row workgroup, radix plan) are listed in DESIGN.md; all but the radix-16 plans (7, 8)
+ * give bitwise identical results. */
 dns_status dns_ens_tuning(dns_ens *e, int m, int *variant, double *ms, int *n);
 /* Device bytes held by the ensemble. */
 size_t dns_ens_device_bytes(const dns_ens *e);

@@ -360,8 +360,10 @@ class Ensemble:
     (N, Re, K0, cflnum, seed) tuples; training pairs every ``pair_every`` steps on the
     N / ``filter_ratio`` grid, channels (filtered omega, u, w, subgrid term).
 
-    ``autotune``: True times the launch configurations of each size and keeps the fastest;
-    ``configuration=v`` runs configuration v (0..3) on every size without timing."""
+    ``autotune``: True (or 1) times the full step of each launch configuration whose results
+    are bitwise the default's and keeps the fastest of each size; 2 also times the radix-16
+    plans (results then agree with single runs to roundoff); ``configuration=v`` runs
+    configuration v (0..8) on every size without timing."""
 
     def __init__(self, members, pair_every: int = 20, filter_ratio: int = 8, max_pairs: int = 100,
                  use_graph: bool = True, device: int = 0, autotune: bool = False,
@@ -374,7 +376,8 @@ class Ensemble:
         self._arr = arr
         cfg = EnsConfig(len(self.members), arr, int(pair_every), int(filter_ratio), int(max_pairs),
                         1 if use_graph else 0, int(device),
-                        2 + int(configuration) if configuration is not None else (1 if autotune else 0))
+                        3 + int(configuration) if configuration is not None
+                        else (0 if not autotune else (2 if int(autotune) == 2 else 1)))
         h = _P()
         st = L.dns_ens_create(ctypes.byref(cfg), ctypes.byref(h))
         if st:
@@ -448,10 +451,10 @@ class Ensemble:
 
     def tuning(self, m: int):
         """(chosen launch configuration, [ms of each configuration]) for member m's size."""
-        v, n = ctypes.c_int(), ctypes.c_int(8)
-        ms = (ctypes.c_double * 8)()
+        v, n = ctypes.c_int(), ctypes.c_int(16)
+        ms = (ctypes.c_double * 16)()
         self._check(load_library().dns_ens_tuning(self._h, m, ctypes.byref(v), ms, ctypes.byref(n)))
-        return v.value, [ms[i] for i in range(min(8, n.value))]
+        return v.value, [ms[i] for i in range(min(16, n.value))]
 
 
 def _fptr(a: np.ndarray):

@@ -31,6 +31,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 #include "dns_big.h"
 #include "fft_lds.hpp"
@@ -69,25 +70,56 @@ constexpr bool big_opq(int pass)
 
 __device__ __forceinline__ double gam(int iz, int N) { return (iz < (N >> 1)) ? double(iz) : double(iz - N); }
 
-// split M = L1 x L2 and the pass tiles: C3 | L2 (pass 3), C4 | L1 (pass 4), C5 | L2 (pass 5)
+// Line lengths M = 3N/2 built for the multi-pass path: the sizes the parity tests force
+// through it (N = 64 .. 1024), the reference's large grids (N = 8192, 9216, 16384, 20480:
+// dns_cuda.cu:309-310, color_cuda.cu:493) and the 7-smooth family around them (N = 3072,
+// 6144, 10240, 12288, 24576, 29400).  Everything below -- the split, the pass tiles and
+// the size dispatch -- is derived from this one list.
+constexpr int kBigM[] = {96,   192,   384,   768,   1536,  4608,  6144,  9216,
+                         12288, 13824, 15360, 18432, 24576, 30720, 36864, 44100};
+constexpr int kNumBigM = sizeof(kBigM) / sizeof(kBigM[0]);
+
+constexpr bool smooth7_m(int m)
+{
+    for (int p : {2, 3, 5, 7})
+        while (m % p == 0) m /= p;
+    return m == 1;
+}
+// split M = L1 x L2 with L1 the largest divisor of M not above sqrt(M) (the squarest
+// split: both line FFTs stay small enough for LDS), and the pass tiles C4 | L1 (k1 lines
+// per pass-4 workgroup) and C5 | L2 (columns per pass-5 workgroup)
+constexpr int split_l1(int m)
+{
+    int best = 1;
+    for (int d = 1; d * d <= m; ++d)
+        if (m % d == 0) best = d;
+    return best;
+}
+constexpr int split_c4(int l1) { return l1 % 4 == 0 ? 4 : (l1 % 3 == 0 ? 3 : (l1 % 2 == 0 ? 2 : 1)); }
+constexpr int split_c5(int l2)
+{
+    return l2 % 8 == 0 ? 8 : (l2 % 14 == 0 ? 14 : (l2 % 4 == 0 ? 4 : (l2 % 6 == 0 ? 6 : 2)));
+}
 template <int M_>
-struct Split;
-#define BIG_SPLIT(M_, L1_, L2_, C3_, C4_, C5_)                                                                  \
-    template <>                                                                                                 \
-    struct Split<M_> {                                                                                          \
-        static constexpr int L1 = L1_, L2 = L2_, C3 = C3_, C4 = C4_, C5 = C5_;                                  \
-        static_assert(L1 * L2 == M_ && L2 % 2 == 0 && L2 % C3 == 0 && L1 % C4 == 0 && L2 % C5 == 0, "split"); \
-    };
-BIG_SPLIT(96, 8, 12, 4, 4, 4)
-BIG_SPLIT(192, 12, 16, 8, 4, 8)
-BIG_SPLIT(384, 16, 24, 8, 4, 8)
-BIG_SPLIT(768, 24, 32, 8, 4, 8)
-BIG_SPLIT(1536, 32, 48, 8, 4, 8)
-BIG_SPLIT(6144, 64, 96, 8, 4, 8)
-BIG_SPLIT(12288, 96, 128, 8, 4, 8)
-BIG_SPLIT(24576, 128, 192, 8, 4, 8)
-BIG_SPLIT(44100, 210, 210, 6, 3, 14)
-#undef BIG_SPLIT
+struct Split {
+    static constexpr int L1 = split_l1(M_), L2 = M_ / L1, C4 = split_c4(L1), C5 = split_c5(L2);
+    static_assert(smooth7_m(M_) && L1 * L2 == M_ && L2 % 2 == 0 && L1 % C4 == 0 && L2 % C5 == 0, "split");
+    static_assert(L1 <= 256 && L2 <= 256, "line FFTs of at most 256 points (LDS)");
+};
+// M = 44100 = 210 x 210 (pass 4: 3 lines, pass 5: 14 columns), as timed
+static_assert(Split<44100>::L1 == 210 && Split<44100>::C4 == 3 && Split<44100>::C5 == 14, "44100 split");
+
+// f(std::integral_constant<int, M>) for the built M equal to m (hipErrorInvalidValue if none)
+template <class F, int I = 0>
+hipError_t with_big_m(int m, F &&f)
+{
+    if constexpr (I < kNumBigM) {
+        if (m == kBigM[I]) return f(std::integral_constant<int, kBigM[I]>{});
+        return with_big_m<F, I + 1>(m, static_cast<F &&>(f));
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
 
 // adjacent kx per z-direction workgroup: 128 bytes of elements
 template <class TC>
@@ -782,18 +814,7 @@ hipError_t launch_lines_m(const BigReq &r)
 template <class TC>
 hipError_t launch_t(const BigReq &r, int M)
 {
-    switch (M) {
-        case 96: return launch_m<TC, 96>(r);
-        case 192: return launch_m<TC, 192>(r);
-        case 384: return launch_m<TC, 384>(r);
-        case 768: return launch_m<TC, 768>(r);
-        case 1536: return launch_m<TC, 1536>(r);
-        case 6144: return launch_m<TC, 6144>(r);
-        case 12288: return launch_m<TC, 12288>(r);
-        case 24576: return launch_m<TC, 24576>(r);
-        case 44100: return launch_m<TC, 44100>(r);
-        default: return hipErrorInvalidValue;
-    }
+    return with_big_m(M, [&](auto m) { return launch_m<TC, decltype(m)::value>(r); });
 }
 
 // the matrix-core line transforms (cm3, cmh) on the N=29400 grid and on one size the
@@ -810,18 +831,7 @@ hipError_t launch_t_mfma(const BigReq &r, int M)
 
 hipError_t launch_lines(const BigReq &r, int M)
 {
-    switch (M) {
-        case 96: return launch_lines_m<96>(r);
-        case 192: return launch_lines_m<192>(r);
-        case 384: return launch_lines_m<384>(r);
-        case 768: return launch_lines_m<768>(r);
-        case 1536: return launch_lines_m<1536>(r);
-        case 6144: return launch_lines_m<6144>(r);
-        case 12288: return launch_lines_m<12288>(r);
-        case 24576: return launch_lines_m<24576>(r);
-        case 44100: return launch_lines_m<44100>(r);
-        default: return hipErrorInvalidValue;
-    }
+    return with_big_m(M, [&](auto m) { return launch_lines_m<decltype(m)::value>(r); });
 }
 
 template <int M_>
@@ -831,30 +841,19 @@ constexpr int cfl_tiles_of() { return Split<M_>::L1 / Split<M_>::C4; }
 
 int dns_big_supported_m(int M)
 {
-    switch (M) {
-        case 96: case 192: case 384: case 768: case 1536: case 6144: case 12288: case 24576: case 44100:
-            return 1;
-        default:
-            return 0;
-    }
+    return with_big_m(M, [](auto) { return hipSuccess; }) == hipSuccess ? 1 : 0;
 }
 
 int dns_big_mfma_supported_m(int M) { return (M == 1536 || M == 44100) ? 1 : 0; }
 
 int dns_big_cfl_tiles(int M)
 {
-    switch (M) {
-        case 96: return cfl_tiles_of<96>();
-        case 192: return cfl_tiles_of<192>();
-        case 384: return cfl_tiles_of<384>();
-        case 768: return cfl_tiles_of<768>();
-        case 1536: return cfl_tiles_of<1536>();
-        case 6144: return cfl_tiles_of<6144>();
-        case 12288: return cfl_tiles_of<12288>();
-        case 24576: return cfl_tiles_of<24576>();
-        case 44100: return cfl_tiles_of<44100>();
-        default: return 0;
-    }
+    int t = 0;
+    (void)with_big_m(M, [&](auto m) {
+        t = cfl_tiles_of<decltype(m)::value>();
+        return hipSuccess;
+    });
+    return t;
 }
 
 hipError_t dns_big_launch(const BigReq &r)

@@ -38,7 +38,7 @@ struct Group {
     cd *om_c = nullptr, *fn_c = nullptr, *V_c = nullptr, *G_c = nullptr, *tw_c = nullptr;
     float *pairs = nullptr;
     int variant = 0;          // launch configuration of the row / column kernels
-    double tune_ms[4] = {0, 0, 0, 0};
+    double tune_ms[16] = {};  // ms per step of each launch configuration (0: not timed)
 };
 
 }  // namespace
@@ -282,41 +282,65 @@ dns_status refresh_group(dns_ens *e, Group &g)
     return klaunch(e, g.H, col_req(e, g.H, g.M, g.om, g.fn, nullptr, g.V, g.tw, 0, batch_of(g, nullptr)));
 }
 
-// Times one batched row pass + column inverse of each launch configuration of g's size
-// (state untouched: the row pass writes G only, the column inverse rewrites V with the
-// same values) and keeps the fastest.
-dns_status autotune_group(dns_ens *e, Group &g)
+// Times the full batched step (row pass + column kernel: z-forward, CN/AB2, z-inverse) of each
+// launch configuration of g's size on scratch copies of om and fn (2 warm-up + 10 timed
+// steps each; zero CN coefficients, so the copies stay bounded) and keeps the fastest.
+// plans: also the configurations with the radix-16 plan, whose results agree with the
+// default's to roundoff instead of bitwise.  V is rebuilt from the real om afterwards.
+dns_status autotune_group(dns_ens *e, Group &g, bool plans)
 {
     const int nv = dns_kernels_variants(g.H, 0);
-    hipEvent_t a = nullptr, b = nullptr;
-    EHIP(hipEventCreate(&a));
-    EHIP(hipEventCreate(&b));
-    double best = 1e300;
-    for (int v = 0; v < nv; ++v) {
-        LaunchReq rr = row_req(e, g.N, g.H, g.M, g.V, g.G, g.tw, nullptr, LAUNCH_ROW, batch_of(g, nullptr));
-        LaunchReq cr = col_req(e, g.H, g.M, g.om, g.fn, nullptr, g.V, g.tw, 0, batch_of(g, nullptr));
-        rr.variant = cr.variant = v;
-        ESTEP(klaunch(e, g.H, rr));  // warm-up
-        ESTEP(klaunch(e, g.H, cr));
-        EHIP(hipEventRecord(a, e->stream));
-        constexpr int reps = 3;
-        for (int k = 0; k < reps; ++k) {
-            ESTEP(klaunch(e, g.H, rr));
-            ESTEP(klaunch(e, g.H, cr));
-        }
-        EHIP(hipEventRecord(b, e->stream));
-        EHIP(hipEventSynchronize(b));
-        float ms = 0.0f;
-        EHIP(hipEventElapsedTime(&ms, a, b));
-        g.tune_ms[v] = ms / reps;
-        if (g.tune_ms[v] < best) {
-            best = g.tune_ms[v];
-            g.variant = v;
-        }
+    const size_t bytes = sizeof(cd) * (size_t)g.count * g.N * g.H;
+    cd *om = nullptr, *fn = nullptr;
+    EHIP(hipMalloc(&om, bytes));
+    if (hipMalloc(&fn, bytes) != hipSuccess) {
+        (void)hipFree(om);
+        return efail(e, DNS_E_NOMEM, "autotune scratch (%zu bytes)", bytes);
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    return DNS_OK;
+    hipEvent_t a = nullptr, b = nullptr;
+    dns_status st = DNS_OK;
+    double best = 1e300;
+    auto run = [&]() -> dns_status {
+        EHIP(hipMemcpyAsync(om, g.om, bytes, hipMemcpyDeviceToDevice, e->stream));
+        EHIP(hipMemcpyAsync(fn, g.fn, bytes, hipMemcpyDeviceToDevice, e->stream));
+        EHIP(hipEventCreate(&a));
+        EHIP(hipEventCreate(&b));
+        for (int v = 0; v < nv; ++v) {
+            g.tune_ms[v] = 0.0;
+            if (!plans && !dns_kernels_variant_bitwise(v)) continue;
+            LaunchReq rr = row_req(e, g.N, g.H, g.M, g.V, g.G, g.tw, nullptr, LAUNCH_ROW, batch_of(g, nullptr));
+            LaunchReq cr = col_req(e, g.H, g.M, om, fn, g.G, g.V, g.tw, 1, batch_of(g, nullptr));
+            rr.variant = cr.variant = v;
+            constexpr int warm = 2, reps = 10;
+            for (int k = 0; k < warm; ++k) {
+                ESTEP(klaunch(e, g.H, rr));
+                ESTEP(klaunch(e, g.H, cr));
+            }
+            EHIP(hipEventRecord(a, e->stream));
+            for (int k = 0; k < reps; ++k) {
+                ESTEP(klaunch(e, g.H, rr));
+                ESTEP(klaunch(e, g.H, cr));
+            }
+            EHIP(hipEventRecord(b, e->stream));
+            EHIP(hipEventSynchronize(b));
+            float ms = 0.0f;
+            EHIP(hipEventElapsedTime(&ms, a, b));
+            g.tune_ms[v] = ms / reps;
+            if (g.tune_ms[v] < best) {
+                best = g.tune_ms[v];
+                g.variant = v;
+            }
+        }
+        return DNS_OK;
+    };
+    st = run();
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipFree(om);
+    (void)hipFree(fn);
+    if (st != DNS_OK) return st;
+    return refresh_group(e, g);  // the timed column kernels rewrote V from the scratch state
 }
 
 }  // namespace
@@ -342,8 +366,8 @@ extern "C" dns_status dns_ens_create(const dns_ens_config *cfg, dns_ens **out)
     if (cfg->pair_every < 1) return efail(nullptr, DNS_E_ARG, "pair_every must be >= 1");
     if (cfg->filter_ratio < 2) return efail(nullptr, DNS_E_ARG, "filter_ratio must be >= 2");
     if (cfg->max_pairs < 0) return efail(nullptr, DNS_E_ARG, "max_pairs must be >= 0");
-    if (cfg->autotune < 0 || cfg->autotune > 5)
-        return efail(nullptr, DNS_E_ARG, "autotune %d: expected 0, 1 or 2 + configuration (0..3)", cfg->autotune);
+    if (cfg->autotune < 0 || cfg->autotune >= 3 + dns_kernels_variants(256, 0))
+        return efail(nullptr, DNS_E_ARG, "autotune %d: expected 0, 1, 2 or 3 + configuration", cfg->autotune);
     for (int m = 0; m < cfg->nmembers; ++m) {
         const dns_ens_member &mb = cfg->members[m];
         const int N = mb.N;
@@ -479,9 +503,9 @@ extern "C" dns_status dns_ens_init(dns_ens *e)
         }
         EHIP(hipMemcpy(g.sc, sc.data(), sizeof(EnsScalars) * g.count, hipMemcpyHostToDevice));
         ESTEP(refresh_group(e, g));
-        if (e->autotune == 1) ESTEP(autotune_group(e, g));
-        if (e->autotune >= 2) {
-            const int v = e->autotune - 2;
+        if (e->autotune == 1 || e->autotune == 2) ESTEP(autotune_group(e, g, e->autotune == 2));
+        if (e->autotune >= 3) {
+            const int v = e->autotune - 3;
             if (v >= dns_kernels_variants(g.H, 0))
                 return efail(e, DNS_E_ARG, "N=%d has %d launch configuration(s), %d requested", g.N,
                              dns_kernels_variants(g.H, 0), v);

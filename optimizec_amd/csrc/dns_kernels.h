@@ -71,6 +71,9 @@ int dns_kernels_mfma_supported(int H);
 // number of launch configurations of the row / column kernels built for H (variant
 // 0 .. n-1 of LaunchReq); all give bitwise identical results
 int dns_kernels_variants(int H, int fp32);
+// 1 if variant v gives results bitwise those of variant 0 (the same arithmetic per element),
+// 0 if it changes the radix plan (agreement to roundoff)
+int dns_kernels_variant_bitwise(int v);
 int dns_kernels_col_tile(int H);
 hipError_t dns_kernels_launch(int H, const LaunchReq &r);
 hipError_t dns_launch_energy(const dnsfft::cd *om, int N, int H, double *partE, double *partZ, hipStream_t s);

@@ -116,7 +116,9 @@ constexpr bool opq_col(int H)
 {
     return H == 2048 || (std::is_same<TC, cd>::value && H == 1024);
 }
-template <class TC, int H, int C, int T, int MINW>
+// PLAN (the ensemble autotuner's launch variants): bit 0 forces the opaque FFT thread index,
+// bit 1 the radix-16 plan of the line transforms (different rounding: not bitwise equal)
+template <class TC, int H, int C, int T, int MINW, int PLAN = 0>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
                                            const TC *__restrict__ G, TC *__restrict__ V,
                                            const TC *__restrict__ twT, const cd *__restrict__ tw, const ColArgs a,
@@ -182,11 +184,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
             return tw[opaque(e)];
         }
     };
+    constexpr bool OQ = opq_col<TC>(H) || (PLAN & 1) != 0;
+    constexpr int MAXR = (PLAN & 2) ? 16 : 8;
     auto lines = [&](auto inv) {  // the 3C H-point lines in LDS
         if constexpr (TABL)
-            fft_lines<H, LS, 3 * C, T, decltype(inv)::value, 8, opq_col<TC>(H)>(lds, twl, 1);
+            fft_lines<H, LS, 3 * C, T, decltype(inv)::value, MAXR, OQ>(lds, twl, 1);
         else
-            fft_lines<H, LS, 3 * C, T, decltype(inv)::value, 8, opq_col<TC>(H)>(lds, twT, 3);
+            fft_lines<H, LS, 3 * C, T, decltype(inv)::value, MAXR, OQ>(lds, twT, 3);
     };
 
     // registers: the (lo, hi) pair of each owned (column c, k1): iz = k1 and iz = k1 + H
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
         // (FP32, and FP64 at H = 2048), so its index arithmetic stays in the loop instead of
         // being hoisted into registers (FP32 at N = 4096: 60 VGPRs and no spill, column
         // kernel 0.37 -> 0.33 ms; FP64 at H = 1024 is not faster with it)
-        constexpr bool FOPQ = opq_col<TC>(H) && (!std::is_same<TC, cd>::value || H == 2048);
+        constexpr bool FOPQ = (opq_col<TC>(H) && (!std::is_same<TC, cd>::value || H == 2048)) || (PLAN & 1) != 0;
         const int tid = FOPQ ? opaque((int)threadIdx.x) : (int)threadIdx.x;
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
 // wave hold the pair (q, H - q), and one cross-lane exchange gives each its mirror.
 // The three residues of one field pair are transformed together (3 lines).
 // =====================================================================================
-template <class TC, int H, int T, int MODE, int MINW>
+template <class TC, int H, int T, int MODE, int MINW, int PLAN = 0>
 __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_t vstride, TC *__restrict__ G,
                                            const TC *__restrict__ tw, double *__restrict__ cflrow,
                                            double DX, double DZ, BatchArgs bt)
@@ -447,7 +451,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
         }
         __syncthreads();
-        fft_lines<H, LS, 3, T, true, 8, opq_row<TC>(H)>(lds, twl, 1);
+        fft_lines<H, LS, 3, T, true, (PLAN & 2) ? 16 : 8, opq_row<TC>(H) || (PLAN & 1) != 0>(lds, twl, 1);
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
@@ -483,7 +487,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
         // A = P0 + i P1, B = P2), then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
         TC *A = lds;
         TC *B = lds + LS;
-        fft_lines<H, LS, 2, T, false, 8, opq_row<TC>(H)>(lds, twl, 1);
+        fft_lines<H, LS, 2, T, false, (PLAN & 2) ? 16 : 8, opq_row<TC>(H) || (PLAN & 1) != 0>(lds, twl, 1);
         TC *g = G + (size_t)z * H;
         const RT half = RT(0.5);
         for (int k = tid; k < H; k += T) {
@@ -521,7 +525,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
 // instead of one value per line.  Per row the work is k_row's, line for line (3R lines per
 // transform, the same engine and twiddles), so the results are bitwise those of k_row.
 // =====================================================================================
-template <class TC, int H, int T, int R, int MINW>
+template <class TC, int H, int T, int R, int MINW, int PLAN = 0>
 __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, size_t vstride, TC *__restrict__ G,
                                               const TC *__restrict__ tw, BatchArgs bt)
 {
@@ -602,7 +606,7 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
         }
         if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
         __syncthreads();
-        fft_lines<H, LS, 3 * R, T, true>(lds, twl, 1);
+        fft_lines<H, LS, 3 * R, T, true, (PLAN & 2) ? 16 : 8, (PLAN & 1) != 0>(lds, twl, 1);
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int e = tid + j * T;
@@ -627,7 +631,7 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
         __syncthreads();
     }
     // x-forward of each row's product, then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
-    fft_lines<H, LS, 2 * R, T, false>(lds, twl, 1);  // lines (r, 0), (r, 1) are 0 .. 2R - 1
+    fft_lines<H, LS, 2 * R, T, false, (PLAN & 2) ? 16 : 8, (PLAN & 1) != 0>(lds, twl, 1);  // lines (r, 0), (r, 1) are 0 .. 2R - 1
     const RT half = RT(0.5);
     for (int e = tid; e < R * H; e += T) {
         const int rr = e / H, k = e - rr * H;
@@ -848,6 +852,8 @@ struct Cfg;
         static constexpr int C = C_;           \
         static constexpr int TC = TC_;         \
         static constexpr int TR = TR_;         \
+        static constexpr int plan = 0;         \
+        static constexpr int rows = 0;         \
     };
 DNS_CFG(8, 8, 64, 64)
 DNS_CFG(16, 4, 64, 64)
@@ -877,17 +883,34 @@ constexpr int col_min_waves()
 }
 
 // Launch variants (the ensemble's autotuner times them): the column tile and the
-// column-kernel threads scaled by CM, the row-kernel threads by TM.  Every variant does
-// the same arithmetic per element (only which thread does it changes), so results are
-// bitwise those of variant 0.
-template <int H, int CM, int TM>
+// column-kernel threads scaled by CM, the row-kernel threads by TM, PLAN (bit 0: the opaque
+// FFT thread index in both kernels; bit 1: the radix-16 plan of every line transform) and
+// ROWS (rows per row-kernel workgroup at H = 128 .. 512; 0: the default).  Variants without
+// the radix-16 plan do the same arithmetic per element (only which thread does it changes),
+// so their results are bitwise those of variant 0; the radix-16 plans round differently.
+template <int H, int CM, int TM, int PLAN = 0, int ROWS = 0>
 struct VarCfg {
     static constexpr int C = Cfg<H>::C * CM;
     static constexpr int TC = Cfg<H>::TC * CM;
     static constexpr int TR = Cfg<H>::TR * TM;
+    static constexpr int plan = PLAN;
+    static constexpr int rows = ROWS;
 };
 // variants are built for FP64 and H a power of two in 16 .. 512 (the ensemble sizes)
 constexpr bool has_variants(int H) { return H >= 16 && H <= 512 && (H & (H - 1)) == 0; }
+// the variant table: (CM, TM, PLAN, ROWS); the last kNumPlanVariants use the radix-16 plan
+#define DNS_VARIANTS(X)    \
+    X(0, 1, 1, 0, 0)       \
+    X(1, 2, 1, 0, 0)       \
+    X(2, 1, 2, 0, 0)       \
+    X(3, 2, 2, 0, 0)       \
+    X(4, 1, 1, 1, 0)       \
+    X(5, 2, 1, 1, 0)       \
+    X(6, 1, 1, 0, 4)       \
+    X(7, 1, 1, 2, 0)       \
+    X(8, 2, 1, 2, 0)
+constexpr int kNumVariants = 9;
+constexpr int kNumPlanVariants = 2;
 
 template <class TC, int H, class Q>
 hipError_t launch_cfg(const LaunchReq &r);
@@ -897,10 +920,10 @@ hipError_t launch_cfg(const LaunchReq &r);
 // workgroups per CU outweigh the longer runs of V values).  At N = 4096 in FP32, two rows
 // per workgroup (127 KB of LDS, one workgroup per CU) ran 0.50 ms; the one-row kernel with
 // the opaque FFT index (61 VGPRs, two 1024-thread workgroups per CU) runs 0.355 ms.
-template <class TC, int H>
+template <class TC, int H, int ROWS = 0>
 struct MrCfg {
-    static constexpr int R = 2;
-    static constexpr int T = H == 128 ? 128 : 256;
+    static constexpr int R = ROWS ? ROWS : 2;
+    static constexpr int T = (H == 128 ? 128 : 256) * (R / 2);
 };
 // which row kernel the step's row pass runs: row groups at H = 128 .. 512 (the default row
 // threads), one row per workgroup otherwise
@@ -917,19 +940,20 @@ struct Cfg32_2048 {
     static constexpr int C = 1;
     static constexpr int TC = 1024;
     static constexpr int TR = 1024;
+    static constexpr int plan = 0;
+    static constexpr int rows = 0;
 };
 
 template <class TC, int H>
 hipError_t launch_prec(const LaunchReq &r)
 {
     if constexpr (has_variants(H) && std::is_same<TC, cd>::value) {
-        {
-            switch (r.variant) {
-                case 1: return launch_cfg<TC, H, VarCfg<H, 2, 1>>(r);
-                case 2: return launch_cfg<TC, H, VarCfg<H, 1, 2>>(r);
-                case 3: return launch_cfg<TC, H, VarCfg<H, 2, 2>>(r);
-                default: break;
-            }
+        switch (r.variant) {
+#define DNS_VARIANT_CASE(V, CM, TM, PLAN, ROWS) \
+    case V: return launch_cfg<TC, H, VarCfg<H, CM, TM, PLAN, ROWS>>(r);
+            DNS_VARIANTS(DNS_VARIANT_CASE)
+#undef DNS_VARIANT_CASE
+            default: break;
         }
     }
     if constexpr (H == 2048 && std::is_same<TC, cf>::value) return launch_cfg<TC, H, Cfg32_2048>(r);
@@ -947,7 +971,7 @@ hipError_t launch_cfg(const LaunchReq &r)
     const unsigned nb = r.batch.count > 1 ? (unsigned)r.batch.count : 1u;
     switch (r.what) {
         case LAUNCH_COL:
-            hipLaunchKernelGGL((k_col<TC, H, Q::C, Q::TC, WC>), dim3(H / Q::C, nb), dim3(Q::TC), 0, r.stream, r.om,
+            hipLaunchKernelGGL((k_col<TC, H, Q::C, Q::TC, WC, Q::plan>), dim3(H / Q::C, nb), dim3(Q::TC), 0, r.stream, r.om,
                                r.fn, static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col, r.batch);
             break;
         case LAUNCH_ROW:
@@ -956,11 +980,11 @@ hipError_t launch_cfg(const LaunchReq &r)
             // autotuner can choose between the two
             if constexpr (use_row_groups<TC, H, Q>()) {
                 if (r.cflrow) return hipErrorInvalidValue;  // the step's row pass computes no CFL
-                constexpr int RR = MrCfg<TC, H>::R, TT = MrCfg<TC, H>::T;
-                hipLaunchKernelGGL((k_row_mr<TC, H, TT, RR, 1>), dim3(M / RR, nb), dim3(TT), 0, r.stream,
+                constexpr int RR = MrCfg<TC, H, Q::rows>::R, TT = MrCfg<TC, H, Q::rows>::T;
+                hipLaunchKernelGGL((k_row_mr<TC, H, TT, RR, 1, Q::plan>), dim3(M / RR, nb), dim3(TT), 0, r.stream,
                                    static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.batch);
             } else {
-                hipLaunchKernelGGL((k_row<TC, H, Q::TR, 0, WR>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
+                hipLaunchKernelGGL((k_row<TC, H, Q::TR, 0, WR, Q::plan>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
                                    static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
                                    r.DX, r.DZ, r.batch);
             }
@@ -1003,8 +1027,10 @@ hipError_t launch_all(const LaunchReq &r)
 
 int dns_kernels_variants(int H, int fp32)
 {
-    return (!fp32 && has_variants(H) && dns_kernels_supported(H)) ? 4 : 1;
+    return (!fp32 && has_variants(H) && dns_kernels_supported(H)) ? kNumVariants : 1;
 }
+
+int dns_kernels_variant_bitwise(int v) { return (v >= 0 && v < kNumVariants - kNumPlanVariants) ? 1 : 0; }
 
 int dns_kernels_mfma_supported(int H) { return mfma_built(H) ? 1 : 0; }
 

@@ -33,15 +33,16 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_supported_sizes():
-    for N in (16, 32, 64, 128, 256, 512, 768, 1024, 2048, 3072, 4096, 8192, 16384, 29400):
+    for N in (16, 32, 64, 128, 256, 512, 768, 1024, 2048, 3072, 4096, 6144, 8192, 9216, 10240, 12288, 16384,
+              20480, 24576, 29400):
         assert oa.supported_n(N), N
-    for N in (10, 30, 100, 6000, 29404):
+    for N in (10, 30, 100, 6000, 29404, 30000):
         assert not oa.supported_n(N), N
     # fused kernels up to N = 4096; the multi-pass path for the large grids and for
     # the sizes the parity tests force through it
     for N in (4096, 3072, 16):
         assert oa.supported_path(N, oa.PATH_FUSED), N
-    for N in (8192, 16384, 29400):
+    for N in (6144, 8192, 9216, 16384, 20480, 29400):
         assert not oa.supported_path(N, oa.PATH_FUSED), N
     for N in (64, 128, 256, 512, 1024, 4096, 8192, 16384, 29400):
         assert oa.supported_path(N, oa.PATH_MULTIPASS), N

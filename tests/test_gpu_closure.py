@@ -7,6 +7,7 @@ Adam update is FP32 and is compared with the oracle's update of the same gradien
 1e-6.  The closure forcing inside a solver is compared with the same network applied
 to the fields computed on the host (numpy inverse FFT).
 """
+import json
 import os
 import sys
 
@@ -173,3 +174,36 @@ def test_closure_beats_the_plain_coarse_run():
     assert clo["spectrum_rel_err_max"] < 0.4 * plain["spectrum_rel_err_max"]
     assert clo["spectrum_rel_err_rms"] < 0.2
     assert abs(clo["final_energy_ratio"] - 1) < 0.02
+
+
+CONFIG5_DNS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                           "closure_dns_N8192_Re1e6_K08_seed0.npz")
+
+
+def test_config5_closure_against_filtered_dns_8192():
+    """The closure configuration: network trained (4000 Adam steps, batch 64, lr 1e-3) on the
+    pairs of the 192 x N=256 + 64 x N=512 ensemble (2000 steps, pairs every 20), then the
+    coarse N=1024 run (Re=1e6, K0=8, seed 0, 5000 steps at 8 dt_f) with and without it,
+    judged against the 8x-filtered FP64 DNS at N=8192 (tests/golden fixture, written by
+    tests/golden/make_closure_reference.py).  Bars: shell-spectrum relative error <= 5% in
+    rms over k = 1 .. 511 and in max over the resolved shells (at least 1e-12 of the peak
+    shell's energy); the closure beats the plain run in rms and in the tail; the DNS took
+    >= 50x the GPU time of the closure run; the enstrophy history within 1e-6."""
+    from optimizec_amd import closure_eval
+
+    print("config 5: closure N=1024 vs filtered DNS N=8192 (fixture)", flush=True)
+    out = closure_eval.evaluate(coarse=1024, ratio=8, Re=1e6, K0=8.0, seed=0, coarse_steps=5000,
+                                ensemble="config", train_iters=4000, dns_cache=CONFIG5_DNS)
+    plain, clo = out["runs"]["plain"], out["runs"]["closure"]
+    print(json.dumps({"resolved": out["resolved_shells"], "ratio": out["gpu_time_ratio_dns_over_closure_run"],
+                      "plain": {k: plain[k] for k in ("spectrum_rel_err_rms", "spectrum_rel_err_max",
+                                                      "spectrum_rel_err_max_resolved", "spectrum_rel_err_max_tail")},
+                      "closure": {k: clo[k] for k in ("seconds", "spectrum_rel_err_rms", "spectrum_rel_err_max",
+                                                      "spectrum_rel_err_max_resolved", "spectrum_rel_err_max_tail")}}))
+    assert out["dns_from_cache"]
+    assert clo["spectrum_rel_err_rms"] <= 0.05
+    assert clo["spectrum_rel_err_max_resolved"] <= 0.05
+    assert clo["spectrum_rel_err_rms"] < plain["spectrum_rel_err_rms"]
+    assert clo["spectrum_rel_err_max_tail"] < plain["spectrum_rel_err_max_tail"]
+    assert out["gpu_time_ratio_dns_over_closure_run"] >= 50.0
+    assert clo["enstrophy_rel_err_max"] < 1e-6

@@ -1,10 +1,10 @@
 """GPU parity of the multi-pass path (batched four-step line FFTs through HBM, used for
-grids the fused kernels do not cover: N = 8192, 16384, 29400) against the FP64 oracle.
+grids the fused kernels do not cover: N = 6144 ... 29400) against the FP64 oracle.
 
 The path is forced at sizes the oracle follows (N <= 1024) and compared directly with
 the same bars as the fused path; at N = 4096 it is compared with the fused path; at
-N = 8192 a few steps are compared with the oracle; at N = 29400 (44100^2 padded, about
-150 GB of device memory) the check is size-independent: a field whose modes all sit at
+N = 3072, 8192 and 9216 a few steps are compared with the oracle; at N = 20480 and 29400
+(44100^2 padded, 117.6 GB of device memory) the check is size-independent: a field whose modes all sit at
 low wavenumbers evolves the same way on every grid that holds the modes its products
 create, so the N = 29400 run must reproduce the oracle's N = 128 run mode by mode.
 """
@@ -109,11 +109,13 @@ def test_multipass_fp32_against_fp64_oracle(oracle_mod):
     s.close()
 
 
-def test_multipass_8192_short_parity(oracle_mod):
-    """N = 8192 (12288^2 padded; auto-selects the multi-pass path): 2 steps vs the oracle."""
-    N = 8192
-    print("multipass N=8192 vs oracle", flush=True)
-    s = oa.Solver(N, 1e5, 4.0, 0.1)
+@pytest.mark.parametrize("N,path", [(8192, oa.PATH_AUTO), (9216, oa.PATH_AUTO), (3072, MP)])
+def test_multipass_short_parity(oracle_mod, N, path):
+    """N = 8192 (12288^2 padded), N = 9216 (13824^2 = 108 x 128 lines, a size outside the
+    powers of two) and N = 3072 forced onto the multi-pass path (4608 = 64 x 72): 2 steps
+    against the oracle, <= 1e-13."""
+    print(f"multipass N={N} vs oracle", flush=True)
+    s = oa.Solver(N, 1e5, 4.0, 0.1, path=path)
     s.init_ic(oa.IC_K4EXP, 0)
     o = oracle_mod.Oracle(N, 1e5, 4.0, 0.1)
     o.init_config(0)
@@ -150,12 +152,13 @@ def _modes(om, N, K):
     return om[rows, :K]
 
 
-def test_29400_low_mode_evolution_matches_oracle_128(oracle_mod):
-    """N = 29400 (the 44100^2 padded FP64 state on one GPU): 3 steps of a low-mode
-    field agree mode by mode with the oracle's N = 128 run of the same field.  The
-    modes start at |k| <= 4, so after 3 steps every mode sits at |k| <= 32: both grids
-    hold all of them without truncation, and the N = 29400 field outside |k| < 40 is
-    roundoff."""
+@pytest.mark.parametrize("N", [20480, 29400])
+def test_large_low_mode_evolution_matches_oracle_128(oracle_mod, N):
+    """N = 29400 (the 44100^2 padded FP64 state on one GPU) and N = 20480 (30720 = 160 x
+    192 lines): 3 steps of a low-mode field agree mode by mode with the oracle's N = 128
+    run of the same field.  The modes start at |k| <= 4, so after 3 steps every mode sits
+    at |k| <= 32: both grids hold all of them without truncation, and the large-grid field
+    outside |k| < 40 is roundoff."""
     steps, K, N0 = 3, 40, 128
     visc, dt = 0.02, 0.05
     o = oracle_mod.Oracle(N0, 50.0, 4.0, 0.1)
@@ -167,7 +170,6 @@ def test_29400_low_mode_evolution_matches_oracle_128(oracle_mod):
     for _ in range(steps):
         o.one_step(form=1)
     ref = _modes(o.om2(), N0, K)
-    N = 29400
     print(f"N={N} multi-pass low-mode run", flush=True)
     s = oa.Solver(N, 50.0, 4.0, 0.1)
     s.set_om2(_low_mode_field(N))
