@@ -256,6 +256,15 @@ def ensemble_leg(steps: int = 2000) -> dict:
     dev = e.device_bytes()
     s0 = e.scalars(0)
     e.close()
+    # the same run on the default launch configuration, for what the autotuner gains
+    d = oa.Ensemble(members, pair_every=20, filter_ratio=8, max_pairs=steps // 20, configuration=0)
+    d.init()
+    d.synchronize()
+    t0 = time.perf_counter()
+    d.advance(steps)
+    d.synchronize()
+    wall_default = time.perf_counter() - t0
+    d.close()
     # one member of each size alone, for the cost of running the members one by one
     single = {}
     for N in (256, 512):
@@ -283,6 +292,8 @@ def ensemble_leg(steps: int = 2000) -> dict:
             "step_hbm_GBps": round(algo / wall / 1e9, 1),
             "pairs_per_member": count, "coarse_n": [n0, 64], "device_GB": round(dev / 1e9, 2),
             "one_by_one_estimate_s": round(seq, 2), "speedup_vs_one_by_one": round(seq / wall, 2),
+            "default_configuration_seconds": round(wall_default, 3),
+            "autotune_speedup": round(wall_default / wall, 3),
             "autotune": {f"N{N}": {"configuration": v, "ms_per_step_by_configuration": [round(x, 4) for x in ms]}
                          for N, (v, ms) in tuning.items()},
             "member0_final": {"it": s0.it, "t": s0.t, "dt": s0.dt}}

@@ -433,9 +433,27 @@ __device__ __forceinline__ void stage_mfma(C *buf, const C *__restrict__ tw, int
     const int n = lane & 15, q = lane >> 4;
     const int x0 = TWO ? 4 * q : 4 * (q & 1);  // first of this lane's 4 inputs
 
-    // DFT-matrix operands (per lane, per stage): W_R^e = W_L^{e L / R} from the table
+    // DFT-matrix operands (per lane, per stage): W_R^e = W_L^{e L / R} from the table, one
+    // load per distinct (x, y) of the lane (4 per row tile; R = 16: row tile 1 from row tile
+    // 0, W_16^{x (y + 8)} = (-1)^x W_16^{x y})
     mf_bf16x8 ah[2], al[2];
     mf_f16x8 af[2];
+    C wxy[2][4];
+#pragma unroll
+    for (int t = 0; t < (TWO ? 2 : 1); ++t) {
+        const int y = 8 * t + (n >> 1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int x = x0 + j;
+            C w{0.0f, 0.0f};
+            if (R == 16 && t == 1) {
+                w = (x & 1) ? C{-wxy[0][j].x, -wxy[0][j].y} : wxy[0][j];
+            } else if (x < R && y < R) {
+                w = tw[opaque(((x * y) % R) * (L / R) * twmul)];
+            }
+            wxy[t][j] = w;
+        }
+    }
 #pragma unroll
     for (int t = 0; t < (TWO ? 2 : 1); ++t) {
         const int y = 8 * t + (n >> 1), c = n & 1;
@@ -444,7 +462,7 @@ __device__ __forceinline__ void stage_mfma(C *buf, const C *__restrict__ tw, int
             const int x = x0 + (i >> 1), d = i & 1;
             float coef = 0.0f;
             if (x < R && y < R) {
-                const C w = tw[opaque(((x * y) % R) * (L / R) * twmul)];
+                const C w = wxy[t][i >> 1];
                 const float wi = INV ? -w.y : w.y;
                 coef = (c == 0) ? (d == 0 ? w.x : -wi) : (d == 0 ? wi : w.x);
             }
