@@ -96,6 +96,7 @@ struct dns_handle {
     cd *om = nullptr, *fn = nullptr, *tw = nullptr;
     void *V = nullptr, *G = nullptr;  // transform precision (cd or cf)
     void *twT = nullptr;        // twiddles in the transform precision (== tw in FP64)
+    void *twH = nullptr;        // fused path: W_H^j = twT[3 j], j < H, contiguous (transform precision)
     cd *Vd = nullptr;           // diagnostics (FP64): 2 z-inverted fields; host staging (lazy)
     double *phys = nullptr;     // diagnostics: one M x M physical plane (lazy)
     double *cflrow = nullptr;   // per-row CFL maxima (M)
@@ -426,6 +427,7 @@ dns_status refresh_v(dns_handle *h)
     r.V = h->V;
     r.tw = h->tw;
     r.twT = h->twT;
+    r.twH = h->twH;
     r.col = inverse_args(h, 4, f4, (size_t)h->M * h->H);
     dns_status st = launch(h, -1, r);
     if (st == DNS_OK) h->v_valid = true;
@@ -744,6 +746,21 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
         if ((e = hipMemcpy(h->twT, twf.data(), sizeof(dnsfft::cf) * h->M, hipMemcpyHostToDevice)) != hipSuccess)
             return bail(fail(h, DNS_E_HIP, "twiddle upload: %s", hipGetErrorString(e)));
     }
+    if (!h->big) {
+        const size_t ebytes = h->fp32 ? sizeof(dnsfft::cf) : sizeof(cd);
+        std::vector<unsigned char> th(ebytes * (size_t)h->H);
+        for (int j = 0; j < h->H; ++j) {
+            if (h->fp32) {
+                const dnsfft::cf v{float(tw[3 * j].x), float(tw[3 * j].y)};
+                std::memcpy(th.data() + ebytes * j, &v, ebytes);
+            } else {
+                std::memcpy(th.data() + ebytes * j, &tw[3 * j], ebytes);
+            }
+        }
+        if ((st = dev_alloc(h, &h->twH, th.size(), "W_H twiddles")) != DNS_OK) return bail(st);
+        if ((e = hipMemcpy(h->twH, th.data(), th.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return bail(fail(h, DNS_E_HIP, "twiddle upload: %s", hipGetErrorString(e)));
+    }
     if ((e = hipMemset(h->om, 0, sizeof(cd) * nspec)) != hipSuccess ||
         (e = hipMemset(h->fn, 0, sizeof(cd) * nspec)) != hipSuccess)
         return bail(fail(h, DNS_E_HIP, "state clear: %s", hipGetErrorString(e)));
@@ -763,7 +780,7 @@ extern "C" void dns_destroy(dns_handle *h)
     for (hipEvent_t ev : h->event_pool) (void)hipEventDestroy(ev);
     if (h->twT == h->tw) h->twT = nullptr;
     if (h->big) h->phys = nullptr;  // a view of buffer B
-    void *ptrs[] = {h->om,  h->fn,     h->V,   h->G,    h->tw,  h->twT, h->Vd,   h->phys,  h->cflrow,
+    void *ptrs[] = {h->om,  h->fn,     h->V,   h->G,    h->tw,  h->twT, h->twH, h->Vd,   h->phys,  h->cflrow,
                     h->partEZ, h->red, h->redu, h->BA,   h->BB,  nullptr,  h->twN, h->caz, h->cbz,
                     h->fext, h->cfields, h->chp, h->cpart, h->cscale};
     for (void *p : ptrs)
@@ -844,6 +861,7 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
         r.Gw = h->G;
         r.tw = h->tw;
         r.twT = h->twT;
+        r.twH = h->twH;
         r.cflrow = nullptr;
         r.DX = r.DZ = 2.0 * PI / double(h->N);
         dns_status st = launch(h, KT_ROW, r);
@@ -858,6 +876,7 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
         c.V = h->V;
         c.tw = h->tw;
         c.twT = h->twT;
+        c.twH = h->twH;
         c.col.do_forward = 1;
         c.col.nfields = 4;
         for (int i = 0; i < 4; ++i) c.col.fields[i] = f4[i];

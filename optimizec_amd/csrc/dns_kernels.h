@@ -52,6 +52,8 @@ struct LaunchReq {
     void *V;                   // cd or cf
     const dnsfft::cd *tw;      // FP64 twiddles W_M^t
     const void *twT;           // twiddles of the transform precision
+    const void *twH;           // W_H^j = twT[3 j] (j < H) stored contiguously, or null (then read
+                               // from twT with stride 3)
     ColArgs col;
     // row kernels
     size_t vstride;

@@ -122,7 +122,7 @@ template <class TC, int H, int C, int T, int MINW, int PLAN = 0>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
                                            const TC *__restrict__ G, TC *__restrict__ V,
                                            const TC *__restrict__ twT, const cd *__restrict__ tw, const ColArgs a,
-                                           BatchArgs bt)
+                                           BatchArgs bt, const TC *__restrict__ twH)
 {
     constexpr int N = 2 * H;
     constexpr int M = 3 * H;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
     const cd wm1 = tw[1], wm2 = tw[2];
     if constexpr (TABL) {
         for (int j = tid; j < H; j += T) {
-            twl[j] = twT[3 * j];
+            twl[j] = twH ? twH[j] : twT[3 * j];
             if constexpr (!F64 && TAB) twd[j] = tw[3 * j];
         }
         __syncthreads();
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
 template <class TC, int H, int T, int MODE, int MINW, int PLAN = 0>
 __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_t vstride, TC *__restrict__ G,
                                            const TC *__restrict__ tw, double *__restrict__ cflrow,
-                                           double DX, double DZ, BatchArgs bt)
+                                           double DX, double DZ, BatchArgs bt, const TC *__restrict__ twH)
 {
     {
         const size_t b = blockIdx.y;
@@ -384,7 +384,8 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
     // complex multiply), so no global load inside the transforms waits behind the
     // prefetched inputs (vector memory loads complete in order)
     const TC wm1 = tw[1], wm2 = tw[2];
-    for (int j = tid; j < H; j += T) twl[j] = tw[3 * j];
+    // (contiguous W_H table when given: one 16-byte line share per entry instead of three)
+    for (int j = tid; j < H; j += T) twl[j] = twH ? twH[j] : tw[3 * j];
     __syncthreads();
     auto wM = [&](int e) {  // component-wise selects (struct selects go through scratch)
         const int q3 = e / 3, r3 = e - 3 * q3;
@@ -972,7 +973,8 @@ hipError_t launch_cfg(const LaunchReq &r)
     switch (r.what) {
         case LAUNCH_COL:
             hipLaunchKernelGGL((k_col<TC, H, Q::C, Q::TC, WC, Q::plan>), dim3(H / Q::C, nb), dim3(Q::TC), 0, r.stream, r.om,
-                               r.fn, static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col, r.batch);
+                               r.fn, static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col, r.batch,
+                               static_cast<const TC *>(r.twH));
             break;
         case LAUNCH_ROW:
             // H = 128 .. 512 at the default row threads: row groups (k_row_mr); the variants
@@ -986,13 +988,13 @@ hipError_t launch_cfg(const LaunchReq &r)
             } else {
                 hipLaunchKernelGGL((k_row<TC, H, Q::TR, 0, WR, Q::plan>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
                                    static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
-                                   r.DX, r.DZ, r.batch);
+                                   r.DX, r.DZ, r.batch, static_cast<const TC *>(r.twH));
             }
             break;
         case LAUNCH_ROW_CFL:
             hipLaunchKernelGGL((k_row<TC, H, Q::TR, 1, WR>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
                                static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
-                               r.DX, r.DZ, r.batch);
+                               r.DX, r.DZ, r.batch, static_cast<const TC *>(r.twH));
             break;
         case LAUNCH_ROW_PHYS:
             hipLaunchKernelGGL((k_row_phys<H, Q::TR>), dim3(M), dim3(Q::TR), 0, r.stream, r.Va, r.Vb, r.tw,
