@@ -94,12 +94,12 @@ def host_facts() -> dict:
 
 
 def oracle_host_bytes(N: int) -> float:
-    """Host bytes the FP64 oracle holds during one advective step at base grid N: om2, fnm1,
-    UC (3 M x NK complex), UR (3 M^2 real), and the step's 4 spectra, 4 physical planes and
-    two M x NK complex planes."""
+    """Host bytes the FP64 oracle touches during start + one advective step at base grid N:
+    om2, fnm1 and FN, the velocity planes of NEXTDT (2 M^2 real), the step's one spectrum,
+    three physical planes and one M x NK complex plane (UC is allocated but never touched)."""
     H, M, NK = N // 2, 3 * N // 2, 3 * N // 4 + 1
     nc = N * H
-    return 16.0 * (2 * nc + 3 * M * NK + 4 * nc + 2 * M * NK) + 8.0 * (3 * M * M + 4 * M * M)
+    return 16.0 * (3 * nc + nc + M * NK) + 8.0 * (2 * M * M + 3 * M * M)
 
 
 def cpu_baseline(N: int, steps: int, gpu: dict):
@@ -136,11 +136,11 @@ def cpu_baseline(N: int, steps: int, gpu: dict):
             "sample": f"N={N} FP64 advective-form oracle, config run steps 1-{steps} (NEXTDT at it=1, "
                       f"{UPDATE}), median of the {len(secs)} step bodies; {wall:.1f} s in total",
             "host": facts, "parity": parity,
-            "n29400": {"status": "not run",
+            "n29400": {"status": "not run in the bench",
                        "reason": f"the oracle holds about {big_need:.0f} GB of host arrays at N={BIG['N']} "
                                  f"(MemAvailable {facts.get('MemAvailable_GB')} GB) and takes minutes per step; "
-                                 "the multi-pass path is checked against it at N=64..1024 and on "
-                                 "low-mode states at N=29400 by the GPU tests"}}
+                                 "the N=29400 configuration is compared with it (start + 1 step, <= 1e-13) "
+                                 "by tests/test_gpu_multipass.py::test_29400_config_one_step_against_oracle"}}
 
 
 def big_leg(N: int) -> dict:

@@ -312,12 +312,11 @@ static void columns_c2c(const fplan *p, int M, int nk, cplx *a, int sign)
 
 static fplan *plan_for(int M);
 
-void or_c2r_2d(int M, const double *uc_in, double *ur)
+/* C2R of a [M][nk] half-plane whose content is destroyed (it is the column pass's buffer) */
+static void c2r_2d_inplace(int M, cplx *tmp, double *ur)
 {
     const int nk = M / 2 + 1;
     const fplan *p = plan_for(M);
-    cplx *tmp = (cplx *)malloc(sizeof(cplx) * (size_t)M * nk);
-    memcpy(tmp, uc_in, sizeof(cplx) * (size_t)M * nk);
     columns_c2c(p, M, nk, tmp, +1);
     const int npairs = (M + 1) / 2;
 #pragma omp parallel
@@ -355,6 +354,14 @@ void or_c2r_2d(int M, const double *uc_in, double *ur)
         free(z);
         free(work);
     }
+}
+
+void or_c2r_2d(int M, const double *uc_in, double *ur)
+{
+    const int nk = M / 2 + 1;
+    cplx *tmp = (cplx *)malloc(sizeof(cplx) * (size_t)M * nk);
+    memcpy(tmp, uc_in, sizeof(cplx) * (size_t)M * nk);
+    c2r_2d_inplace(M, tmp, ur);
     free(tmp);
 }
 
@@ -1059,42 +1066,70 @@ void or_velocity_from_om2(or_state *S)
             vel_from_om(S->om2[(size_t)iz * H + ix], ix, iz, N, &cu[(size_t)iz * H + ix], &cw[(size_t)iz * H + ix]);
     cplx *g = (cplx *)malloc(sizeof(cplx) * (size_t)M * NK);
     scatter_compact(S, cu, g);
-    or_c2r_2d(M, (const double *)g, S->ur);
+    c2r_2d_inplace(M, g, S->ur);
     scatter_compact(S, cw, g);
-    or_c2r_2d(M, (const double *)g, S->ur + (size_t)M * M);
+    c2r_2d_inplace(M, g, S->ur + (size_t)M * M);
     free(g);
     free(cu);
     free(cw);
+}
+
+/* The advective FN = -FFT(u d_x omega + w d_z omega)/M^2 on the dealiased set.  Memory-lean
+ * (it also runs at N = 29400, where one M x M plane is 15.6 GB): each field's spectrum is
+ * built straight into the 3/2-grid buffer, which the in-place C2R consumes; the product
+ * P = u d_x omega + w d_z omega is formed in two passes over three planes, P = u ox, then
+ * P = P + w oz -- with -ffp-contract=off that is the same two roundings of each product and
+ * one of the sum as evaluating the expression at once. */
+static void field_spec_compact(const or_state *S, int f, cplx *dst)
+{
+    const int N = S->N, H = S->H;
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < N; ++iz) {
+        const double gz = gam_of(iz, N);
+        for (int ix = 0; ix < H; ++ix) {
+            const size_t i = (size_t)iz * H + ix;
+            const cplx om = S->om2[i];
+            cplx u, w;
+            switch (f) {
+                case 0: vel_from_om(om, ix, iz, N, &u, &w); dst[i] = u; break;
+                case 1: vel_from_om(om, ix, iz, N, &u, &w); dst[i] = w; break;
+                case 2: dst[i] = cmk(-(double)ix * om.im, (double)ix * om.re); break; /* i alfa om  */
+                default: dst[i] = cmk(-gz * om.im, gz * om.re); break;             /* i gamma om */
+            }
+        }
+    }
 }
 
 void or_nonlinear_adv(or_state *S, double *fn_out)
 {
     const int N = S->N, H = S->H, M = S->M, NK = S->NK;
     const size_t nc = (size_t)N * H, plane = (size_t)M * M;
-    cplx *spec = (cplx *)malloc(sizeof(cplx) * nc * 4);
-    for (int iz = 0; iz < N; ++iz) {
-        const double gz = gam_of(iz, N);
-        for (int ix = 0; ix < H; ++ix) {
-            const size_t i = (size_t)iz * H + ix;
-            const cplx om = S->om2[i];
-            vel_from_om(om, ix, iz, N, &spec[i], &spec[nc + i]);
-            spec[2 * nc + i] = cmk(-(double)ix * om.im, (double)ix * om.re); /* i alfa om  */
-            spec[3 * nc + i] = cmk(-gz * om.im, gz * om.re);                 /* i gamma om */
-        }
-    }
-    double *phys = (double *)malloc(sizeof(double) * plane * 4);
+    cplx *spec = (cplx *)malloc(sizeof(cplx) * nc);
+    double *P = (double *)malloc(sizeof(double) * plane);
+    double *A = (double *)malloc(sizeof(double) * plane);
+    double *B = (double *)malloc(sizeof(double) * plane);
     cplx *g = (cplx *)malloc(sizeof(cplx) * (size_t)M * NK);
-    for (int f = 0; f < 4; ++f) {
-        scatter_compact(S, spec + (size_t)f * nc, g);
-        or_c2r_2d(M, (const double *)g, phys + (size_t)f * plane);
-    }
-    double *P = phys; /* reuse plane 0 */
+    /* fields 0 (u), 1 (w), 2 (d_x omega), 3 (d_z omega) */
+    field_spec_compact(S, 0, spec);
+    scatter_compact(S, spec, g);
+    c2r_2d_inplace(M, g, A);
+    field_spec_compact(S, 2, spec);
+    scatter_compact(S, spec, g);
+    c2r_2d_inplace(M, g, B);
 #pragma omp parallel for schedule(static)
-    for (size_t i = 0; i < plane; ++i) {
-        const double u = phys[i], w = phys[plane + i], ox = phys[2 * plane + i], oz = phys[3 * plane + i];
-        P[i] = u * ox + w * oz;
-    }
+    for (size_t i = 0; i < plane; ++i) P[i] = A[i] * B[i];
+    field_spec_compact(S, 1, spec);
+    scatter_compact(S, spec, g);
+    c2r_2d_inplace(M, g, A);
+    field_spec_compact(S, 3, spec);
+    scatter_compact(S, spec, g);
+    c2r_2d_inplace(M, g, B);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < plane; ++i) P[i] = P[i] + A[i] * B[i];
+    free(A);
+    free(B);
     or_r2c_2d(M, P, (double *)g);
+    free(P);
     const double divxz = 1.0 / ((double)M * (double)M);
     cplx *fn = (cplx *)fn_out;
     for (int iz = 0; iz < N; ++iz) {
@@ -1109,7 +1144,6 @@ void or_nonlinear_adv(or_state *S, double *fn_out)
         }
     }
     free(g);
-    free(phys);
     free(spec);
 }
 
