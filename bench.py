@@ -292,6 +292,19 @@ def ensemble_leg(steps: int = 2000) -> dict:
     dev = e.device_bytes()
     s0 = e.scalars(0)
     e.close()
+    # the whole run as one graph launch (use_graph = 2): the first advance captures it (from
+    # it = 0), the second captures the continuation, the third replays that graph
+    g = oa.Ensemble(members, pair_every=20, filter_ratio=8, max_pairs=steps // 20, use_graph=2,
+                    configuration=0)
+    g.init()
+    g.synchronize()
+    one_launch = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        g.advance(steps)
+        g.synchronize()
+        one_launch.append(time.perf_counter() - t0)
+    g.close()
     # the same run on the default launch configuration, for what the autotuner gains
     d = oa.Ensemble(members, pair_every=20, filter_ratio=8, max_pairs=steps // 20, configuration=0)
     d.init()
@@ -329,6 +342,9 @@ def ensemble_leg(steps: int = 2000) -> dict:
             "pairs_per_member": count, "coarse_n": [n0, 64], "device_GB": round(dev / 1e9, 2),
             "one_by_one_estimate_s": round(seq, 2), "speedup_vs_one_by_one": round(seq / wall, 2),
             "default_configuration_seconds": round(wall_default, 3),
+            "one_graph_launch": {"steps": steps, "first_run_with_capture_s": round(one_launch[0], 3),
+                                 "replay_s": round(one_launch[2], 3),
+                                 "ms_per_step": round(1e3 * one_launch[2] / steps, 4)},
             "autotune_speedup": round(wall_default / wall, 3),
             "autotune": {f"N{N}": {"configuration": v, "ms_per_step_by_configuration": [round(x, 4) for x in ms]}
                          for N, (v, ms) in tuning.items()},

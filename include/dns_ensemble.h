@@ -54,7 +54,11 @@ typedef struct {
     int                   pair_every;    /* steps between NEXTDT calls and training pairs (e.g. 20) */
     int                   filter_ratio;  /* coarse grid n = N / filter_ratio (e.g. 8)             */
     int                   max_pairs;     /* training pairs kept per member (device memory)        */
-    int                   use_graph;     /* 1: capture a block of pair_every steps in a HIP graph */
+    int                   use_graph;     /* 1: capture a block of pair_every steps in a HIP graph
+                                            and replay it; 2: capture each dns_ens_advance call
+                                            whole (all its blocks, the first block's NEXTDT after
+                                            step 1 included) into one graph, launched once and
+                                            reused by later calls of the same length; 0: none */
     int                   device;
     int                   autotune;      /* 1: dns_ens_init times the full step (row pass +
                                             column kernel) of each launch configuration whose

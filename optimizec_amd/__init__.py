@@ -366,7 +366,7 @@ class Ensemble:
     configuration v (0..8) on every size without timing."""
 
     def __init__(self, members, pair_every: int = 20, filter_ratio: int = 8, max_pairs: int = 100,
-                 use_graph: bool = True, device: int = 0, autotune: bool = False,
+                 use_graph: int = 1, device: int = 0, autotune: bool = False,
                  configuration: int | None = None):
         L = load_library()
         self.members = [tuple(m) for m in members]
@@ -375,7 +375,7 @@ class Ensemble:
             arr[i] = EnsMember(int(N), float(Re), float(K0), float(cfl), int(seed))
         self._arr = arr
         cfg = EnsConfig(len(self.members), arr, int(pair_every), int(filter_ratio), int(max_pairs),
-                        1 if use_graph else 0, int(device),
+                        int(use_graph), int(device),
                         3 + int(configuration) if configuration is not None
                         else (0 if not autotune else (2 if int(autotune) == 2 else 1)))
         h = _P()

@@ -6,7 +6,8 @@
 // block of pair_every steps -- snapshot, steps, training pair, NEXTDT -- is a fixed
 // sequence of launches: it is captured once into a HIP graph and replayed.  The first
 // block of a run differs (NEXTDT also after step 1, as in the single-run loop) and is
-// issued directly.
+// issued directly.  With use_graph = 2 a whole dns_ens_advance call -- every block of it,
+// the first one included -- is one graph, launched once.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -56,6 +57,12 @@ struct dns_ens {
     long long steps_done = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // use_graph == 2: one graph for a whole dns_ens_advance call (run_blocks blocks, the
+    // first one starting at it = 0 when run_first), launched once
+    hipGraph_t run_graph = nullptr;
+    hipGraphExec_t run_exec = nullptr;
+    int run_blocks = 0;
+    bool run_first = false;
     size_t bytes = 0;
     std::vector<void *> allocs;
     std::string err;
@@ -244,8 +251,43 @@ void drop_graph(dns_ens *e)
 {
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
+    if (e->run_exec) (void)hipGraphExecDestroy(e->run_exec);
+    if (e->run_graph) (void)hipGraphDestroy(e->run_graph);
     e->exec = nullptr;
     e->graph = nullptr;
+    e->run_exec = nullptr;
+    e->run_graph = nullptr;
+    e->run_blocks = 0;
+}
+
+// the whole advance of `blocks` blocks (the first from it = 0 when first) as one graph
+dns_status capture_run(dns_ens *e, int blocks, bool first)
+{
+    if (e->run_exec) (void)hipGraphExecDestroy(e->run_exec);
+    if (e->run_graph) (void)hipGraphDestroy(e->run_graph);
+    e->run_exec = nullptr;
+    e->run_graph = nullptr;
+    e->run_blocks = 0;
+    EHIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    dns_status st = DNS_OK;
+    for (int b = 0; b < blocks && st == DNS_OK; ++b) st = issue_block(e, first && b == 0);
+    hipGraph_t g = nullptr;
+    hipError_t x = hipStreamEndCapture(e->stream, &g);
+    if (st != DNS_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (x != hipSuccess) return efail(e, DNS_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(x));
+    e->run_graph = g;
+    x = hipGraphInstantiate(&e->run_exec, e->run_graph, nullptr, nullptr, 0);
+    if (x != hipSuccess) {
+        (void)hipGraphDestroy(e->run_graph);
+        e->run_graph = nullptr;
+        return efail(e, DNS_E_HIP, "hipGraphInstantiate (whole run): %s", hipGetErrorString(x));
+    }
+    e->run_blocks = blocks;
+    e->run_first = first;
+    return DNS_OK;
 }
 
 dns_status capture_block(dns_ens *e)
@@ -376,6 +418,8 @@ extern "C" dns_status dns_ens_create(const dns_ens_config *cfg, dns_ens **out)
     if (cfg->pair_every < 1) return efail(nullptr, DNS_E_ARG, "pair_every must be >= 1");
     if (cfg->filter_ratio < 2) return efail(nullptr, DNS_E_ARG, "filter_ratio must be >= 2");
     if (cfg->max_pairs < 0) return efail(nullptr, DNS_E_ARG, "max_pairs must be >= 0");
+    if (cfg->use_graph < 0 || cfg->use_graph > 2)
+        return efail(nullptr, DNS_E_ARG, "use_graph %d: expected 0, 1 or 2", cfg->use_graph);
     if (cfg->autotune < 0 || cfg->autotune >= 3 + dns_kernels_variants(256, 0))
         return efail(nullptr, DNS_E_ARG, "autotune %d: expected 0, 1, 2 or 3 + configuration", cfg->autotune);
     for (int m = 0; m < cfg->nmembers; ++m) {
@@ -400,7 +444,7 @@ extern "C" dns_status dns_ens_create(const dns_ens_config *cfg, dns_ens **out)
     e->pair_every = cfg->pair_every;
     e->ratio = cfg->filter_ratio;
     e->max_pairs = cfg->max_pairs;
-    e->use_graph = cfg->use_graph ? 1 : 0;
+    e->use_graph = cfg->use_graph;
     e->autotune = cfg->autotune;
     e->members.assign(cfg->members, cfg->members + cfg->nmembers);
     auto bail = [&](dns_status st) {
@@ -550,6 +594,15 @@ extern "C" dns_status dns_ens_advance(dns_ens *e, int nsteps)
         return efail(e, DNS_E_ARG, "nsteps=%d must be a non-negative multiple of pair_every=%d", nsteps, e->pair_every);
     EHIP(hipSetDevice(e->device));
     const int blocks = nsteps / e->pair_every;
+    if (e->use_graph == 2 && blocks > 0) {
+        // the whole advance in one graph launch (captured once per block count and start)
+        const bool first = e->steps_done == 0;
+        if (!e->run_exec || e->run_blocks != blocks || e->run_first != first) ESTEP(capture_run(e, blocks, first));
+        EHIP(hipGraphLaunch(e->run_exec, e->stream));
+        e->steps_done += blocks * e->pair_every;
+        e->pairs_emitted += blocks;
+        return DNS_OK;
+    }
     for (int b = 0; b < blocks; ++b) {
         if (e->steps_done == 0) {
             ESTEP(issue_block(e, true));

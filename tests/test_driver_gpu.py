@@ -66,23 +66,26 @@ def test_driver_multipass_lines_match_fused(tmp_path):
             assert m[key] == f[key], (it, key, m[key], f[key])
 
 
-def test_driver_mixed_precision_knob(tmp_path):
-    """DNS_PREC=fp32 runs the mixed-precision step from the command line: the ITERATION
-    lines follow the FP64 run's (T, DT, CN, CFLM to 1e-6 relative; NEXTDT runs in FP64) and
-    the run announces its precision."""
+@pytest.mark.parametrize("prec,banner,tol", [("fp32", "precision: FP32 transforms, FP64 state", 1e-6),
+                                             ("bf16x3", "precision: FP32 transforms with 3xBF16 matrix-core", 1e-6),
+                                             ("fp16", "precision: FP32 transforms with FP16 matrix-core", 1e-4)])
+def test_driver_mixed_precision_knob(tmp_path, prec, banner, tol):
+    """DNS_PREC=fp32 / bf16x3 / fp16 runs the mixed-precision step from the command line: the
+    ITERATION lines follow the FP64 run's (T, DT, CN, CFLM to 1e-6 relative, 1e-4 with the
+    FP16 products; NEXTDT runs in FP64) and the run announces its precision."""
     args = ["512", "1e4", "4", "200", "0.1", "50", "0"]
     env = dict(os.environ, DNS_IC="k4exp", DNS_SEED="2")
     r64 = subprocess.run([DRIVER, *args], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
-    r32 = subprocess.run([DRIVER, *args], cwd=tmp_path, env=dict(env, DNS_PREC="fp32"), capture_output=True,
+    r32 = subprocess.run([DRIVER, *args], cwd=tmp_path, env=dict(env, DNS_PREC=prec), capture_output=True,
                          text=True, timeout=120)
     assert r64.returncode == 0 and r32.returncode == 0, r64.stderr + r32.stderr
-    assert "precision: FP32 transforms" in r32.stdout and "precision:" not in r64.stdout
+    assert banner in r32.stdout and "precision:" not in r64.stdout
     a, b = _iterations(r64.stdout), _iterations(r32.stdout)
     assert sorted(a) == sorted(b) == [1, 50, 100, 150, 200]
     for it in a:
         for key in ("T", "DT", "CN", "CFLM"):
             x, y = float(a[it][key]), float(b[it][key])
-            assert abs(y - x) <= 1e-6 * abs(x), (it, key, x, y)
+            assert abs(y - x) <= tol * abs(x), (it, key, x, y)
 
 
 def test_driver_restart_roundtrip(tmp_path):

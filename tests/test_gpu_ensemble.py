@@ -34,10 +34,11 @@ def _standalone(m, steps, record=()):
     return s, states
 
 
-@pytest.mark.parametrize("use_graph,autotune", [(True, False), (False, False), (True, True)])
+@pytest.mark.parametrize("use_graph,autotune", [(1, False), (0, False), (1, True), (2, False)])
 def test_members_bitwise_equal_single_runs(use_graph, autotune):
-    """Also with the autotuner: every launch configuration does the same arithmetic per
-    element, so the chosen one changes nothing in the results."""
+    """Blocks replayed from a graph (1), issued directly (0) or the whole advance captured as
+    one graph launch (2); also with the autotuner: every launch configuration it times does
+    the same arithmetic per element, so the chosen one changes nothing in the results."""
     print(f"ensemble vs single runs, graph={use_graph}, autotune={autotune}", flush=True)
     e = oa.Ensemble(MEMBERS, pair_every=EVERY, filter_ratio=RATIO, max_pairs=8, use_graph=use_graph,
                     autotune=autotune)
