@@ -519,6 +519,143 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
 }
 
 // =====================================================================================
+// Persistent row kernel (the step's MODE 0 at H = 2048): RPW rows per workgroup, one
+// workgroup per CU.  Per row the work is k_row's line for line (the same arithmetic, so the
+// results are bitwise equal), but the W_H table is filled once per workgroup and the next
+// row's first field pair is loaded while the current row's forward transform runs.  Rows keep
+// the XCD-aware order: workgroup b runs on XCD b mod 8 and takes rows x (M / 8) + i (NW / 8)
+// + b / 8, so at any time the 32 workgroups of an XCD work on consecutive rows that share
+// the 128-byte lines of the column-major V in that XCD's L2.
+// =====================================================================================
+template <class TC, int H, int T, int RPW>
+__global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t vstride, TC *__restrict__ G,
+                                                const TC *__restrict__ tw, const TC *__restrict__ twH)
+{
+    using RT = decltype(TC::x);
+    constexpr int LS = padded_len(H);
+    constexpr int PJ = H / T;  // input pairs per lane
+    constexpr int PO = H / T;  // outputs per thread and residue
+    static_assert(T % 64 == 0 && H % T == 0, "whole wavefronts; pair slots tile the half row");
+    __shared__ TC lds[3 * LS];
+    __shared__ TC twl[H];
+    constexpr int M = 3 * H;
+    int tid = threadIdx.x;
+    int lane = tid & 63;
+    const int nw = gridDim.x;  // a multiple of 8 with RPW * nw = M (checked by the launcher)
+    auto row_of = [&](int i) {
+        const int b = blockIdx.x, x = b & 7, c = b >> 3;
+        return x * (M / 8) + i * (nw / 8) + c;
+    };
+    const TC w3_1 = tw[2 * H], w3_2 = tw[H];
+    const TC zero{RT(0), RT(0)};
+    const TC wm1 = tw[1], wm2 = tw[2];
+    for (int j = tid; j < H; j += T) twl[j] = twH ? twH[j] : tw[3 * j];
+    __syncthreads();
+    auto wM = [&](int e) {
+        const int q3 = e / 3, r3 = e - 3 * q3;
+        const TC t = twl[q3];
+        const TC m = mul(t, TC{r3 == 1 ? wm1.x : wm2.x, r3 == 1 ? wm1.y : wm2.y});
+        return TC{r3 == 0 ? t.x : m.x, r3 == 0 ? t.y : m.y};
+    };
+    TC vel[3][PO];
+    TC ra0[PJ], rb0[PJ], ra1[PJ], rb1[PJ];
+    auto issue = [&](const TC *fa, TC *a, TC *b) {
+        const TC *fb = fa + vstride;
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const int q = (tid >> 6) * 32 + (lane & 31) + j * (T / 2);
+            const int k = lane < 32 ? q : (q == 0 ? H / 2 : H - q);
+            TC va = zero, vb = zero;
+            if (q < H / 2) {
+                va = fa[(size_t)k * M];
+                vb = fb[(size_t)k * M];
+            }
+            a[j] = va;
+            b[j] = vb;
+        }
+    };
+    issue(V + row_of(0), ra0, rb0);
+#pragma unroll 1
+    for (int i = 0; i < RPW; ++i) {
+        // the thread index through an empty asm per row: the index arithmetic of the row
+        // stays inside the loop instead of occupying registers across it
+        tid = opaque((int)threadIdx.x);
+        lane = tid & 63;
+        const int z = row_of(i);
+        const TC *row = V + z;
+#pragma unroll
+        for (int ph = 0; ph < 2; ++ph) {
+#pragma unroll
+            for (int j = 0; j < PJ; ++j) {
+                const int q = (tid >> 6) * 32 + (lane & 31) + j * (T / 2);
+                const bool valid = q < H / 2;
+                const bool hi = lane >= 32;
+                const int k = !hi ? q : (q == 0 ? H / 2 : H - q);
+                const TC av = ph == 0 ? ra0[j] : ra1[j], bv = ph == 0 ? rb0[j] : rb1[j];
+                TC am{__shfl_xor(av.x, 32, 64), __shfl_xor(av.y, 32, 64)};
+                TC bm{__shfl_xor(bv.x, 32, 64), __shfl_xor(bv.y, 32, 64)};
+                if (k == H / 2) {
+                    am = av;
+                    bm = bv;
+                }
+                if (valid) {
+                    TC zlo, zhi;
+                    if (k == 0) {
+                        zlo = TC{av.x, bv.x};
+                        zhi = zero;
+                    } else {
+                        zlo = TC{av.x - bv.y, av.y + bv.x};
+                        zhi = TC{am.x + bm.y, bm.x - am.y};
+                    }
+                    lds[pidx(k)] = add(zlo, zhi);
+                    lds[LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_1)), wM(k));
+                    lds[2 * LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_2)), wM(2 * k));
+                }
+            }
+            if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
+            __syncthreads();
+            fft_lines<H, LS, 3, T, true, 8, opq_row<TC>(H)>(lds, twl, 1);
+#pragma unroll
+            for (int j = 0; j < PO; ++j) {
+                const int x1 = tid + j * T;
+                if (ph == 0) {
+#pragma unroll
+                    for (int n2 = 0; n2 < 3; ++n2) vel[n2][j] = lds[n2 * LS + pidx(x1)];
+                } else {
+                    RT P[3];
+#pragma unroll
+                    for (int n2 = 0; n2 < 3; ++n2) {
+                        const TC v = lds[n2 * LS + pidx(x1)];
+                        P[n2] = vel[n2][j].x * v.x + vel[n2][j].y * v.y;
+                    }
+                    lds[pidx(x1)] = TC{P[0], P[1]};
+                    lds[LS + pidx(x1)] = TC{P[2], RT(0)};
+                }
+            }
+            // the next row's first pair, into the registers this row's pairs have freed: in
+            // flight during the forward transform
+            if (ph == 1 && i + 1 < RPW) issue(V + row_of(i + 1), ra0, rb0);
+            __syncthreads();
+        }
+        TC *A = lds;
+        TC *B = lds + LS;
+        fft_lines<H, LS, 2, T, false, 8, opq_row<TC>(H)>(lds, twl, 1);
+        TC *g = G + (size_t)z * H;
+        const RT half = RT(0.5);
+        for (int k = tid; k < H; k += T) {
+            const TC za = A[pidx(k)];
+            const TC zm = conj(A[pidx(k == 0 ? 0 : H - k)]);
+            const TC s = add(za, zm), d = sub(za, zm);
+            const TC F0{half * s.x, half * s.y};
+            const TC F1{half * d.y, -half * d.x};
+            const TC F2 = B[pidx(k)];
+            g[k] = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
+        }
+        __syncthreads();  // the next row writes the lines this loop reads
+    }
+}
+
+// =====================================================================================
 // Row kernel over row groups (H = 128 .. 512): R consecutive rows per workgroup, MODE 0
 // only (the step).  V is column-major, so the R rows' values of one mode kx sit side by side (16 R
 // bytes in FP64); lane l of a wave takes row z0 + (l mod 32) mod R and mode slot
@@ -986,6 +1123,17 @@ hipError_t launch_cfg(const LaunchReq &r)
                 hipLaunchKernelGGL((k_row_mr<TC, H, TT, RR, 1, Q::plan>), dim3(M / RR, nb), dim3(TT), 0, r.stream,
                                    static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.batch);
             } else {
+                if constexpr (std::is_same<TC, cd>::value && H == 2048 && std::is_same<Q, Cfg<H>>::value) {
+                    // FP64 at N = 4096: the persistent row kernel, 256 workgroups x 24 rows
+                    constexpr int RPW = 24, NW = M / RPW;
+                    static_assert(NW % 8 == 0 && NW * RPW == M, "whole rows per workgroup, whole XCD sets");
+                    if (nb == 1 && !r.cflrow) {
+                        hipLaunchKernelGGL((k_row_p<TC, H, Q::TR, RPW>), dim3(NW), dim3(Q::TR), 0, r.stream,
+                                           static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT,
+                                           static_cast<const TC *>(r.twH));
+                        break;
+                    }
+                }
                 hipLaunchKernelGGL((k_row<TC, H, Q::TR, 0, WR, Q::plan>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
                                    static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
                                    r.DX, r.DZ, r.batch, static_cast<const TC *>(r.twH));
