@@ -58,7 +58,7 @@ struct LaunchReq {
     // row kernels
     size_t vstride;
     void *Gw;
-    double *cflrow;
+    double *cflrow;            // LAUNCH_ROW_CFL: per-row CFL maxima (LAUNCH_ROW: must be null)
     double DX, DZ;
     const dnsfft::cd *Va, *Vb;
     double *out0, *out1;

@@ -337,7 +337,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
 
 // =====================================================================================
 // Row kernel: x-direction work on one z-row of the padded grid.
-// MODE 0: full nonlinear row (inverse of u+iw and wx+iwz, product, CFL max, forward).
+// MODE 0: full nonlinear row (inverse of u+iw and wx+iwz, product, forward); the step's
+//         row pass, no CFL (the step's dt comes from NEXTDT).
 // MODE 1: velocity only (inverse of u+iw, CFL max) -- NEXTDT (dns_cuda.cu:485-547).
 //
 // The x-inverse of two real fields a, b packs Z = a + i b on the full M-point
@@ -462,8 +463,10 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
                     for (int n2 = 0; n2 < 3; ++n2) {
                         const TC v = lds[n2 * LS + pidx(x1)];
                         vel[n2][j] = v;
-                        const double cfl = fabs(double(v.x)) / DX + fabs(double(v.y)) / DZ;
-                        cmax = cfl > cmax ? cfl : cmax;
+                        if constexpr (MODE == 1) {
+                            const double cfl = fabs(double(v.x)) / DX + fabs(double(v.y)) / DZ;
+                            cmax = cfl > cmax ? cfl : cmax;
+                        }
                     }
                 } else {
                     // the real product P = u d_x omega + w d_z omega, written straight into
@@ -502,7 +505,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
         }
     }
 
-    if (cflrow) {
+    if (MODE == 1 && cflrow) {
         // wavefront max, then across the workgroup's waves
         for (int off = 32; off > 0; off >>= 1) {
             const double o = __shfl_xor(cmax, off, 64);
@@ -1114,11 +1117,12 @@ hipError_t launch_cfg(const LaunchReq &r)
                                static_cast<const TC *>(r.twH));
             break;
         case LAUNCH_ROW:
+            // the step's row pass computes no CFL (LAUNCH_ROW_CFL does)
+            if (r.cflrow) return hipErrorInvalidValue;
             // H = 128 .. 512 at the default row threads: row groups (k_row_mr); the variants
             // with doubled row threads keep one row per workgroup, so the ensemble's
             // autotuner can choose between the two
             if constexpr (use_row_groups<TC, H, Q>()) {
-                if (r.cflrow) return hipErrorInvalidValue;  // the step's row pass computes no CFL
                 constexpr int RR = MrCfg<TC, H, Q::rows>::R, TT = MrCfg<TC, H, Q::rows>::T;
                 hipLaunchKernelGGL((k_row_mr<TC, H, TT, RR, 1, Q::plan>), dim3(M / RR, nb), dim3(TT), 0, r.stream,
                                    static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.batch);
@@ -1127,7 +1131,7 @@ hipError_t launch_cfg(const LaunchReq &r)
                     // FP64 at N = 4096: the persistent row kernel, 256 workgroups x 24 rows
                     constexpr int RPW = 24, NW = M / RPW;
                     static_assert(NW % 8 == 0 && NW * RPW == M, "whole rows per workgroup, whole XCD sets");
-                    if (nb == 1 && !r.cflrow) {
+                    if (nb == 1) {
                         hipLaunchKernelGGL((k_row_p<TC, H, Q::TR, RPW>), dim3(NW), dim3(Q::TR), 0, r.stream,
                                            static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT,
                                            static_cast<const TC *>(r.twH));
