@@ -336,6 +336,154 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
 }
 
 // =====================================================================================
+// Persistent column kernel (the step's column pass at H = 2048, FP64): CPW columns per
+// workgroup, one workgroup per CU.  Per column the work is k_col's line for line (so the
+// results are bitwise equal); the twiddle table is filled once per workgroup and the next
+// column's G values are gathered while the current column's last inverse transform runs.
+// =====================================================================================
+template <int H, int T, int CPW>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1))) void k_col_p(
+    cd *__restrict__ om, cd *__restrict__ fn, const cd *__restrict__ G, cd *__restrict__ V, const cd *__restrict__ twT,
+    const cd *__restrict__ tw, const ColArgs a, const cd *__restrict__ twH)
+{
+    using TC = cd;
+    constexpr int N = 2 * H;
+    constexpr int M = 3 * H;
+    constexpr int PJ = H / T;
+    constexpr int GJ = 3 * H / T;  // G values per thread and column
+    static_assert(H % T == 0, "whole half columns per thread");
+    const double VT = a.VT, c2 = a.c2, c3 = a.c3;
+    constexpr int LS = padded_len(H);
+    __shared__ TC lds[3 * LS];
+    __shared__ TC twl[H];
+    const int nw = gridDim.x;  // a multiple of 8 with CPW * nw = H (checked by the launcher)
+    auto col_of = [&](int i) {
+        const int b = blockIdx.x, x = b & 7, c = b >> 3;
+        return x * (H / 8) + i * (nw / 8) + c;
+    };
+    const cd wm1 = tw[1], wm2 = tw[2];
+    for (int j = threadIdx.x; j < H; j += T) twl[j] = twH ? twH[j] : twT[3 * j];
+    __syncthreads();
+    auto wM = [&](int e) -> cd {
+        const int q3 = e / 3, r3 = e - 3 * q3;
+        const cd t = twl[q3];
+        const cd m = mul(t, cd{r3 == 1 ? wm1.x : wm2.x, r3 == 1 ? wm1.y : wm2.y});
+        return cd{r3 == 0 ? t.x : m.x, r3 == 0 ? t.y : m.y};
+    };
+    auto lines = [&](auto inv) { fft_lines<H, LS, 3, T, decltype(inv)::value, 8, true>(lds, twl, 1); };
+    // the G values of a column (z = tid + r T), in registers: copied component-wise, since a
+    // whole-struct copy between global memory and this array becomes a memcpy that keeps the
+    // array in scratch (and makes the prefetch wait for its loads)
+    cd gx[GJ];
+#pragma unroll
+    for (int r = 0; r < GJ; ++r) {
+        const cd v = G[(size_t)(threadIdx.x + r * T) * H + col_of(0)];
+        gx[r] = mk(v.x, v.y);
+    }
+#pragma unroll 1
+    for (int i = 0; i < CPW; ++i) {
+        const int tid = opaque((int)threadIdx.x);
+        const int kx0 = col_of(i);
+        cd xlo[PJ], xhi[PJ];
+#pragma unroll
+        for (int r = 0; r < GJ; ++r) {
+            const int z = tid + r * T;
+            const int z1 = z / 3, z2 = z - 3 * z1;
+            lds[z2 * LS + pidx(z1)] = mk(gx[r].x, gx[r].y);
+        }
+        __syncthreads();
+        lines(std::false_type{});
+        const cd w3a = tw[2 * H];
+        const cd w3b = tw[H];
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const int k1 = tid + j * T;
+            const cd F0 = lds[pidx(k1)];
+            const cd F1 = mul(lds[LS + pidx(k1)], wM(k1));
+            const cd F2 = mul(lds[2 * LS + pidx(k1)], wM(2 * k1));
+            xlo[j] = add(F0, add(F1, F2));
+            xhi[j] = add(F0, add(mul(F1, w3a), mul(F2, w3b)));
+        }
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const int k1 = tid + j * T;
+            const int kx = kx0;
+            const double ax = double(kx);
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int iz = k1 + half * H;
+                const double gz = gamma_of(iz, N);
+                const double K2 = ax * ax + gz * gz;
+                const cd X = half ? xhi[j] : xlo[j];
+                cd FN = mk(-X.x * a.divxz, -X.y * a.divxz);
+                const size_t io = (size_t)kx * N + iz;
+                if (a.fext) FN = add(FN, a.fext[io]);
+                if (iz == H) FN = mk(0.0, 0.0);
+                const cd o = om[io];
+                const cd f = fn[io];
+                const double ARG = VT * K2;
+                const double DEN = 1.0 + ARG;
+                const double c1 = 1.0 - ARG;
+                const cd num = mk(c1 * o.x + c2 * FN.x + c3 * f.x, c1 * o.y + c2 * FN.y + c3 * f.y);
+                const cd on = mk(num.x / DEN, num.y / DEN);
+                om[io] = on;
+                fn[io] = FN;
+                if (half)
+                    xhi[j] = on;
+                else
+                    xlo[j] = on;
+            }
+        }
+        __syncthreads();
+        double ik_lo[PJ], ik_hi[PJ];
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const int k1 = tid + j * T;
+            const double ax = double(kx0);
+            ik_lo[j] = inv_k2_of(ax, gamma_of(k1, N));
+            ik_hi[j] = inv_k2_of(ax, gamma_of(k1 + H, N));
+        }
+        const int nf = a.nfields;
+#pragma unroll 1
+        for (int fi = 0; fi < nf; ++fi) {
+            const int f = a.fields[fi];
+            const int tid = opaque((int)threadIdx.x);
+#pragma unroll
+            for (int j = 0; j < PJ; ++j) {
+                const int k1 = tid + j * T;
+                const int kx = kx0;
+                const double ax = double(kx);
+                const double glo = gamma_of(k1, N), ghi = gamma_of(k1 + H, N);
+                const double iglo = (kx == 0) ? inv_g_of(k1, glo) : 0.0;
+                const double ighi = (kx == 0) ? inv_g_of(k1 + H, ghi) : 0.0;
+                const cd slo = field_spec(f, xlo[j], kx, k1, ax, glo, ik_lo[j], iglo);
+                const cd shi = field_spec(f, xhi[j], kx, k1 + H, ax, ghi, ik_hi[j], ighi);
+                lds[pidx(k1)] = add(slo, shi);
+                lds[LS + pidx(k1)] = mulc(add(slo, mulc(shi, tw[2 * H])), wM(k1));
+                lds[2 * LS + pidx(k1)] = mulc(add(slo, mulc(shi, tw[H])), wM(2 * k1));
+            }
+            // the next column's G values, in flight during this column's last transform
+            if (fi == nf - 1 && i + 1 < CPW) {
+                const int kn = col_of(i + 1);
+#pragma unroll
+                for (int r = 0; r < GJ; ++r) {
+                    const cd v = G[(size_t)(tid + r * T) * H + kn];
+                    gx[r] = mk(v.x, v.y);
+                }
+            }
+            __syncthreads();
+            lines(std::true_type{});
+            TC *out = V + (size_t)fi * a.vstride + (size_t)kx0 * M;
+            for (int e = tid; e < M; e += T) {
+                const int z1 = e / 3, z2 = e - 3 * z1;
+                out[e] = lds[z2 * LS + pidx(z1)];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// =====================================================================================
 // Row kernel: x-direction work on one z-row of the padded grid.
 // MODE 0: full nonlinear row (inverse of u+iw and wx+iwz, product, forward); the step's
 //         row pass, no CFL (the step's dt comes from NEXTDT).
@@ -1112,6 +1260,18 @@ hipError_t launch_cfg(const LaunchReq &r)
     const unsigned nb = r.batch.count > 1 ? (unsigned)r.batch.count : 1u;
     switch (r.what) {
         case LAUNCH_COL:
+            if constexpr (H == 2048 && std::is_same<TC, cd>::value && std::is_same<Q, Cfg<H>>::value) {
+                // FP64 at N = 4096, the step's column pass: the persistent column kernel,
+                // 256 workgroups x 8 columns
+                constexpr int CPW = 8, NW = H / CPW;
+                static_assert(NW % 8 == 0 && NW * CPW == H && Q::C == 1, "whole columns per workgroup");
+                if (nb == 1 && !r.batch.coef && r.col.do_forward && r.col.nfields >= 1) {
+                    hipLaunchKernelGGL((k_col_p<H, Q::TC, CPW>), dim3(NW), dim3(Q::TC), 0, r.stream, r.om, r.fn,
+                                       static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col,
+                                       static_cast<const TC *>(r.twH));
+                    break;
+                }
+            }
             hipLaunchKernelGGL((k_col<TC, H, Q::C, Q::TC, WC, Q::plan>), dim3(H / Q::C, nb), dim3(Q::TC), 0, r.stream, r.om,
                                r.fn, static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col, r.batch,
                                static_cast<const TC *>(r.twH));
