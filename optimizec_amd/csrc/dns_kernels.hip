@@ -116,6 +116,22 @@ constexpr bool opq_col(int H)
 {
     return H == 2048 || (std::is_same<TC, cd>::value && H == 1024);
 }
+// In-place line transforms (fft_lines_to_pos / fft_lines_from_pos: one barrier per stage)
+// where they timed faster.  The persistent FP64 row kernel (k_row_p) takes them always
+// (0.748 -> 0.713 ms at N = 4096).  Timed slower or equal elsewhere on one box, so off: the
+// other row kernels' inverse + forward pairs (FP32 row kernel at N = 4096 0.345 -> 0.351 ms,
+// ensemble step 1.23 -> 1.27 ms) and the column kernels' forward lines, whose G rows are then
+// gathered in position order (FP64 0.499 -> 0.512 ms, FP32 0.319 -> 0.329 ms).
+template <class TC>
+constexpr bool ip_row(int H)
+{
+    return false;
+}
+template <class TC>
+constexpr bool ip_col(int H)
+{
+    return false;
+}
 // PLAN (the ensemble autotuner's launch variants): bit 0 forces the opaque FFT thread index,
 // bit 1 the radix-16 plan of the line transforms (different rounding: not bitwise equal)
 template <class TC, int H, int C, int T, int MINW, int PLAN = 0>
@@ -192,15 +208,24 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
         else
             fft_lines<H, LS, 3 * C, T, decltype(inv)::value, MAXR, OQ>(lds, twT, 3);
     };
+    // the forward lines take their input in position order (in place, see fft_lines_from_pos):
+    // position p of a residue line holds z1 = zpos(p); G is read one value per row anyway
+    auto lines_fwd = [&]() {
+        if constexpr (TABL)
+            fft_lines_from_pos<H, LS, 3 * C, T, false, MAXR, OQ, ip_col<TC>(H)>(lds, twl, 1);
+        else
+            fft_lines_from_pos<H, LS, 3 * C, T, false, MAXR, OQ, ip_col<TC>(H)>(lds, twT, 3);
+    };
+    auto zpos = [](int p) { return lines_in_place<TC, ip_col<TC>(H)>() ? ip_idx(H, p, MAXR) : p; };
 
     // registers: the (lo, hi) pair of each owned (column c, k1): iz = k1 and iz = k1 + H
     cd xlo[PJ], xhi[PJ];
 
     if (a.do_forward) {
         for (int e = tid; e < 3 * C * H; e += T) {
-            const int c = e % C, z = e / C;
-            const int z1 = z / 3, z2 = z - 3 * z1;
-            lds[(z2 * C + c) * LS + pidx(z1)] = G[(size_t)z * H + kx0 + c];
+            const int c = e % C, q = e / C;
+            const int p = q / 3, z2 = q - 3 * p;
+            lds[(z2 * C + c) * LS + pidx(p)] = G[(size_t)(3 * zpos(p) + z2) * H + kx0 + c];
         }
         // om, fn of the owned modes (lo: iz = k1, hi: iz = k1 + H), in flight during the
         // forward transforms when the twiddles come from LDS
@@ -218,7 +243,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
             }
         }
         __syncthreads();
-        lines(std::false_type{});
+        lines_fwd();
         // radix-3 combine in FP64 (the FFT outputs are widened)
         const cd w3a = tw[2 * H];  // W_3^2
         const cd w3b = tw[H];      // W_3^4 = W_3
@@ -371,13 +396,19 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1))) void k_c
         return cd{r3 == 0 ? t.x : m.x, r3 == 0 ? t.y : m.y};
     };
     auto lines = [&](auto inv) { fft_lines<H, LS, 3, T, decltype(inv)::value, 8, true>(lds, twl, 1); };
-    // the G values of a column (z = tid + r T), in registers: copied component-wise, since a
-    // whole-struct copy between global memory and this array becomes a memcpy that keeps the
-    // array in scratch (and makes the prefetch wait for its loads)
+    // the forward lines take their input in position order (in place): slot q = 3 p + z2 of
+    // the gather holds G row 3 ip_idx(p) + z2
+    auto grow = [](int q) {
+        const int p = q / 3, z2 = q - 3 * p;
+        return 3 * (ip_col<cd>(H) ? ip_idx(H, p) : p) + z2;
+    };
+    // the G values of a column (slots q = tid + r T), in registers: copied component-wise,
+    // since a whole-struct copy between global memory and this array becomes a memcpy that
+    // keeps the array in scratch (and makes the prefetch wait for its loads)
     cd gx[GJ];
 #pragma unroll
     for (int r = 0; r < GJ; ++r) {
-        const cd v = G[(size_t)(threadIdx.x + r * T) * H + col_of(0)];
+        const cd v = G[(size_t)grow(threadIdx.x + r * T) * H + col_of(0)];
         gx[r] = mk(v.x, v.y);
     }
 #pragma unroll 1
@@ -392,7 +423,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1))) void k_c
             lds[z2 * LS + pidx(z1)] = mk(gx[r].x, gx[r].y);
         }
         __syncthreads();
-        lines(std::false_type{});
+        fft_lines_from_pos<H, LS, 3, T, false, 8, true, ip_col<cd>(H)>(lds, twl, 1);
         const cd w3a = tw[2 * H];
         const cd w3b = tw[H];
 #pragma unroll
@@ -467,7 +498,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1))) void k_c
                 const int kn = col_of(i + 1);
 #pragma unroll
                 for (int r = 0; r < GJ; ++r) {
-                    const cd v = G[(size_t)(tid + r * T) * H + kn];
+                    const cd v = G[(size_t)grow(tid + r * T) * H + kn];
                     gx[r] = mk(v.x, v.y);
                 }
             }
@@ -601,7 +632,9 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
         }
         __syncthreads();
-        fft_lines<H, LS, 3, T, true, (PLAN & 2) ? 16 : 8, opq_row<TC>(H) || (PLAN & 1) != 0>(lds, twl, 1);
+        // vector lines in place: position p holds x1 = ip_idx(p) (the product below is pointwise
+        // and the forward takes that order back); matrix-core lines in natural order
+        fft_lines_to_pos<H, LS, 3, T, true, (PLAN & 2) ? 16 : 8, opq_row<TC>(H) || (PLAN & 1) != 0, ip_row<TC>(H)>(lds, twl, 1);
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
@@ -639,7 +672,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
         // A = P0 + i P1, B = P2), then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
         TC *A = lds;
         TC *B = lds + LS;
-        fft_lines<H, LS, 2, T, false, (PLAN & 2) ? 16 : 8, opq_row<TC>(H) || (PLAN & 1) != 0>(lds, twl, 1);
+        fft_lines_from_pos<H, LS, 2, T, false, (PLAN & 2) ? 16 : 8, opq_row<TC>(H) || (PLAN & 1) != 0, ip_row<TC>(H)>(lds, twl, 1);
         TC *g = G + (size_t)z * H;
         const RT half = RT(0.5);
         for (int k = tid; k < H; k += T) {
@@ -671,9 +704,11 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
 
 // =====================================================================================
 // Persistent row kernel (the step's MODE 0 at H = 2048): RPW rows per workgroup, one
-// workgroup per CU.  Per row the work is k_row's line for line (the same arithmetic, so the
-// results are bitwise equal), but the W_H table is filled once per workgroup and the next
-// row's first field pair is loaded while the current row's forward transform runs.  Rows keep
+// workgroup per CU.  Per row the work is k_row's, but the line transforms run in place
+// (fft_lines_ip: one barrier per stage instead of two; a DIF inverse leaves x1 = ip_idx(p) at
+// position p, the product is formed position by position and the DIT forward returns
+// natural order, so nothing is permuted), the W_H table is filled once per workgroup and the
+// next row's first field pair is loaded while the current row's forward transform runs.  Rows keep
 // the XCD-aware order: workgroup b runs on XCD b mod 8 and takes rows x (M / 8) + i (NW / 8)
 // + b / 8, so at any time the 32 workgroups of an XCD work on consecutive rows that share
 // the 128-byte lines of the column-major V in that XCD's L2.
@@ -765,7 +800,9 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
             }
             if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
             __syncthreads();
-            fft_lines<H, LS, 3, T, true, 8, opq_row<TC>(H)>(lds, twl, 1);
+            // in place (one barrier per stage): position p of every line holds x1 = ip_idx(p);
+            // the product below is pointwise and the forward DIT takes that order back
+            fft_lines_to_pos<H, LS, 3, T, true, 8, opq_row<TC>(H)>(lds, twl, 1);
 #pragma unroll
             for (int j = 0; j < PO; ++j) {
                 const int x1 = tid + j * T;
@@ -790,7 +827,7 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
         }
         TC *A = lds;
         TC *B = lds + LS;
-        fft_lines<H, LS, 2, T, false, 8, opq_row<TC>(H)>(lds, twl, 1);
+        fft_lines_from_pos<H, LS, 2, T, false, 8, opq_row<TC>(H)>(lds, twl, 1);
         TC *g = G + (size_t)z * H;
         const RT half = RT(0.5);
         for (int k = tid; k < H; k += T) {
@@ -895,7 +932,7 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
         }
         if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
         __syncthreads();
-        fft_lines<H, LS, 3 * R, T, true, (PLAN & 2) ? 16 : 8, (PLAN & 1) != 0>(lds, twl, 1);
+        fft_lines_to_pos<H, LS, 3 * R, T, true, (PLAN & 2) ? 16 : 8, (PLAN & 1) != 0, ip_row<TC>(H)>(lds, twl, 1);
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int e = tid + j * T;
@@ -920,7 +957,7 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
         __syncthreads();
     }
     // x-forward of each row's product, then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
-    fft_lines<H, LS, 2 * R, T, false, (PLAN & 2) ? 16 : 8, (PLAN & 1) != 0>(lds, twl, 1);  // lines (r, 0), (r, 1) are 0 .. 2R - 1
+    fft_lines_from_pos<H, LS, 2 * R, T, false, (PLAN & 2) ? 16 : 8, (PLAN & 1) != 0, ip_row<TC>(H)>(lds, twl, 1);  // lines (r, 0), (r, 1) are 0 .. 2R - 1
     const RT half = RT(0.5);
     for (int e = tid; e < R * H; e += T) {
         const int rr = e / H, k = e - rr * H;

@@ -570,6 +570,188 @@ __device__ __forceinline__ void stage_mfma(C *buf, const C *__restrict__ tw, int
     __syncthreads();
 }
 
+// ---- in-place stages: decimation in frequency (DIF) and in time (DIT) ---------------
+// Stage s of a plan (radices R_0 .. R_{S-1} from pick_radix) has the stride
+// ST_s = L / (R_0 ... R_s).  Butterfly (blk, j), j < ST_s, owns the positions
+// blk R_s ST_s + j + r ST_s (r < R_s) of its line and writes its outputs back to them, so a
+// stage needs one barrier (after it) where a Stockham stage needs two.
+//   DIF, stages s = 0 .. S-1: DFT over r, then output r times W_{R_s ST_s}^{j r}.  Natural
+//        input; index k of the transform ends at position ip_pos(k).
+//   DIT, stages s = S-1 .. 0: input r times W_{R_s ST_s}^{j r}, then DFT over r.  Index k of
+//        the input starts at position ip_pos(k); natural output.  (DIT is DIF transposed,
+//        and the DFT matrix is symmetric.)
+// A DIF inverse followed by a pointwise operation and a DIT forward therefore never
+// permutes anything: position p holds index ip_idx(p) in both.
+__host__ __device__ constexpr int ip_stride(int L, int s, int maxr = 8)
+{
+    return L / (stage_span(L, s, maxr) * stage_radix(L, s, maxr));
+}
+// k = d_0 + R_0 (d_1 + R_1 (d_2 + ...))  ->  position sum_s d_s ST_s
+__host__ __device__ constexpr int ip_pos(int L, int k, int maxr = 8)
+{
+    int pos = 0, m = L;
+    while (m > 1) {
+        const int r = pick_radix(m, maxr);
+        m /= r;
+        pos += (k % r) * m;
+        k /= r;
+    }
+    return pos;
+}
+// the index held at position p (inverse of ip_pos)
+__host__ __device__ constexpr int ip_idx(int L, int p, int maxr = 8)
+{
+    int k = 0, m = L, span = 1;
+    while (m > 1) {
+        const int r = pick_radix(m, maxr);
+        m /= r;
+        k += (p / m) * span;
+        p %= m;
+        span *= r;
+    }
+    return k;
+}
+
+// One in-place butterfly's arithmetic on its R values; w1 = W_BL^j and, for R > 8,
+// w8 = W_BL^{8 j} (forward sign; the inverse takes conjugates).  Powers as in stage().
+template <int R, bool INV, class C>
+__device__ __forceinline__ void twiddle_ip(C *v, C w1, C w8)  // v[r] *= w^r (conj for INV)
+{
+    C wp[R];
+    wp[1] = w1;
+#pragma unroll
+    for (int r = 2; r < R && r < 8; ++r) wp[r] = (r % 2 == 0) ? mul(wp[r / 2], wp[r / 2]) : mul(wp[r - 1], w1);
+    if constexpr (R > 8) {
+        wp[8] = w8;
+#pragma unroll
+        for (int r = 9; r < R; ++r) wp[r] = mul(w8, wp[r - 8]);
+    }
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = INV ? mulc(v[r], wp[r]) : mul(v[r], wp[r]);
+}
+template <int R, bool INV, bool DIT, bool TW, class C>
+__device__ __forceinline__ void bfly_ip(C *v, C w1, C w8)
+{
+    if constexpr (DIT && TW) twiddle_ip<R, INV, C>(v, w1, w8);
+    dft<R, INV, C>(v);
+    if constexpr (!DIT && TW) twiddle_ip<R, INV, C>(v, w1, w8);
+}
+// x returned through an empty asm that also takes dep: whatever is indexed by the result
+// waits for dep to be computed (keeps a DIF butterfly's twiddles from being loaded and
+// powered up while its DFT still holds all its registers)
+template <class C>
+__device__ __forceinline__ int after(int x, C dep)
+{
+    asm volatile("" : "+v"(x) : "v"(dep.x));
+    return x;
+}
+
+template <int L, int LS, int NL, int T, int R, int ST, bool INV, bool DIT, bool OPQ, class C>
+__device__ __forceinline__ void stage_ip(C *buf, const C *__restrict__ tw, int twmul)
+{
+    constexpr int LR = L / R;  // butterflies per line
+    constexpr int NB = NL * LR;
+    constexpr int PER = (NB + T - 1) / T;
+    constexpr int BL = R * ST;
+    constexpr int TWS = L / BL;  // W_BL^s = W_L^{s TWS}
+    const int tid = OPQ ? opaque((int)threadIdx.x) : (int)threadIdx.x;
+    // one butterfly after the other (a thread's later loads cannot pass its earlier stores
+    // to the same buffer; other waves fill the gaps)
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int b = tid + q * T;
+        if ((NB % T) == 0 || b < NB) {
+            const int line = b / LR;
+            const int jj = b - line * LR;
+            const int blk = jj / ST, j = jj - blk * ST;
+            C v[R];
+            {
+                const C *src = buf + line * LS;
+                const int base = blk * BL + j;
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = src[pidx(base + r * ST)];
+            }
+            C w1{}, w8{};
+            if constexpr (DIT || ST == 1) {
+                if constexpr (ST > 1) {
+                    w1 = tw[opaque(j * TWS * twmul)];
+                    if constexpr (R > 8) w8 = tw[opaque(8 * j * TWS * twmul)];
+                }
+                bfly_ip<R, INV, DIT, (ST > 1), C>(v, w1, w8);
+            } else {  // DIF: the twiddles after the DFT, see after()
+                dft<R, INV, C>(v);
+                const int e = after(j * TWS * twmul, v[0]);
+                w1 = tw[e];
+                if constexpr (R > 8) w8 = tw[opaque(8 * e)];
+                twiddle_ip<R, INV, C>(v, w1, w8);
+            }
+            C *dst = buf + line * LS;
+            const int base = blk * BL + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[pidx(base + r * ST)] = v[r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, int LS, int NL, int T, bool INV, bool DIT, int S, int MAXR, bool OPQ, class C>
+__device__ __forceinline__ void stages_ip_from(C *buf, const C *__restrict__ tw, int twmul)
+{
+    constexpr int NSTG = num_stages(L, MAXR);
+    if constexpr (S < NSTG) {
+        constexpr int s = DIT ? NSTG - 1 - S : S;
+        stage_ip<L, LS, NL, T, stage_radix(L, s, MAXR), ip_stride(L, s, MAXR), INV, DIT, OPQ, C>(buf, tw, twmul);
+        stages_ip_from<L, LS, NL, T, INV, DIT, S + 1, MAXR, OPQ, C>(buf, tw, twmul);
+    }
+}
+
+// In-place counterpart of fft_lines (vector stages only): DIT = false transforms natural
+// input into the ip_pos order, DIT = true transforms ip_pos-ordered input into natural
+// order.  Same conventions as fft_lines otherwise (callers barrier before; the engine ends
+// with a barrier).
+template <int L, int LS, int NL, int T, bool INV, bool DIT, int MAXR = 8, bool OPQ = false, class C>
+__device__ __forceinline__ void fft_lines_ip(C *buf, const C *__restrict__ tw, int twmul)
+{
+    static_assert(mfma_kind<C>::value == 0, "in-place stages: vector radices only");
+    static_assert(smooth7(L), "line length must factor into 2, 3, 5, 7");
+    static_assert(LS >= padded_len(L), "line stride must hold the padded line");
+    static_assert(MAXR == 8 || MAXR == 16, "radix plans: 8 or 16");
+    stages_ip_from<L, LS, NL, T, INV, DIT, 0, MAXR, OPQ, C>(buf, tw, twmul);
+}
+
+// Line pairs that only need "position p of the inverse's output is position p of the
+// forward's input" (pointwise work in between): in-place stages (DIF inverse / DIT forward)
+// for vector lines where the caller asks for them (IP), Stockham stages otherwise (natural
+// order on both sides; always for matrix-core lines).  lines_in_place<C, IP>() tells the
+// caller which order the forward's input is in.
+template <class C, bool IP = true>
+__host__ __device__ constexpr bool lines_in_place()
+{
+    return IP && mfma_kind<C>::value == 0;
+}
+template <int L, int LS, int NL, int T, bool INV, int S, int MAXR, bool OPQ, class C>
+__device__ __forceinline__ void stages_from(C *buf, const C *__restrict__ tw, int twmul);
+template <int L, int LS, int NL, int T, bool INV, int MAXR = 8, bool OPQ = false, class C>
+__device__ __forceinline__ void fft_lines(C *buf, const C *__restrict__ tw, int twmul);
+// natural input -> position order (in place) / natural order (Stockham)
+template <int L, int LS, int NL, int T, bool INV, int MAXR = 8, bool OPQ = false, bool IP = true, class C>
+__device__ __forceinline__ void fft_lines_to_pos(C *buf, const C *__restrict__ tw, int twmul)
+{
+    if constexpr (lines_in_place<C, IP>())
+        fft_lines_ip<L, LS, NL, T, INV, false, MAXR, OPQ>(buf, tw, twmul);
+    else
+        fft_lines<L, LS, NL, T, INV, MAXR, OPQ>(buf, tw, twmul);
+}
+// position-order input (in place) / natural input (Stockham) -> natural order
+template <int L, int LS, int NL, int T, bool INV, int MAXR = 8, bool OPQ = false, bool IP = true, class C>
+__device__ __forceinline__ void fft_lines_from_pos(C *buf, const C *__restrict__ tw, int twmul)
+{
+    if constexpr (lines_in_place<C, IP>())
+        fft_lines_ip<L, LS, NL, T, INV, true, MAXR, OPQ>(buf, tw, twmul);
+    else
+        fft_lines<L, LS, NL, T, INV, MAXR, OPQ>(buf, tw, twmul);
+}
+
 template <int L, int LS, int NL, int T, bool INV, int S, int MAXR, bool OPQ, class C>
 __device__ __forceinline__ void stages_from(C *buf, const C *__restrict__ tw, int twmul)
 {
@@ -590,7 +772,7 @@ __device__ __forceinline__ void stages_from(C *buf, const C *__restrict__ tw, in
 // engine ends with a barrier, so results are visible to every thread on return.
 // tw may point to global memory or to an LDS table.  MAXR selects the radix plan (8 or 16,
 // see pick_radix); OPQ keeps each stage's index arithmetic inside the caller's loops.
-template <int L, int LS, int NL, int T, bool INV, int MAXR = 8, bool OPQ = false, class C>
+template <int L, int LS, int NL, int T, bool INV, int MAXR, bool OPQ, class C>
 __device__ __forceinline__ void fft_lines(C *buf, const C *__restrict__ tw, int twmul)
 {
     static_assert(smooth7(L), "line length must factor into 2, 3, 5, 7");
