@@ -608,8 +608,8 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             const int k = !hi ? q : (q == 0 ? H / 2 : H - q);
             const TC av = ph == 0 ? ra0[j] : ra1[j], bv = ph == 0 ? rb0[j] : rb1[j];
             // partner lane (l ^ 32) holds H - k, except in the q = 0 pair (0 and H/2)
-            TC am{__shfl_xor(av.x, 32, 64), __shfl_xor(av.y, 32, 64)};
-            TC bm{__shfl_xor(bv.x, 32, 64), __shfl_xor(bv.y, 32, 64)};
+            TC am{swap_halves(av.x), swap_halves(av.y)};
+            TC bm{swap_halves(bv.x), swap_halves(bv.y)};
             if (k == H / 2) {  // its own mirror
                 am = av;
                 bm = bv;
@@ -778,8 +778,8 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
                 const bool hi = lane >= 32;
                 const int k = !hi ? q : (q == 0 ? H / 2 : H - q);
                 const TC av = ph == 0 ? ra0[j] : ra1[j], bv = ph == 0 ? rb0[j] : rb1[j];
-                TC am{__shfl_xor(av.x, 32, 64), __shfl_xor(av.y, 32, 64)};
-                TC bm{__shfl_xor(bv.x, 32, 64), __shfl_xor(bv.y, 32, 64)};
+                TC am{swap_halves(av.x), swap_halves(av.y)};
+                TC bm{swap_halves(bv.x), swap_halves(bv.y)};
                 if (k == H / 2) {
                     am = av;
                     bm = bv;
@@ -912,8 +912,8 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
             int q;
             const int k = mode_of(j, &q);
             const TC av = ph == 0 ? ra0[j] : ra1[j], bv = ph == 0 ? rb0[j] : rb1[j];
-            TC am{__shfl_xor(av.x, 32, 64), __shfl_xor(av.y, 32, 64)};
-            TC bm{__shfl_xor(bv.x, 32, 64), __shfl_xor(bv.y, 32, 64)};
+            TC am{swap_halves(av.x), swap_halves(av.y)};
+            TC bm{swap_halves(bv.x), swap_halves(bv.y)};
             if (k == H / 2) {
                 am = av;
                 bm = bv;

@@ -89,6 +89,26 @@ __device__ __forceinline__ int opaque(int x)
     return x;
 }
 
+// Value of lane l ^ 32 of the wavefront (the other half).  Doubles go through
+// v_permlane32_swap, a VALU lane swap, instead of the two LDS permutes of a 64-bit
+// __shfl_xor (FP64 row kernels at N = 4096: 0.713 -> 0.706 ms; the ensemble step -1%); the
+// swap of a register with itself leaves lanes 0-31 of the first result and lanes 32-63 of the
+// second holding the other half's values.  Floats keep the one LDS permute (the FP32 row
+// kernel timed 2% slower with the swap).  Blocks of whole wavefronts (lane = threadIdx.x mod 64).
+__device__ __forceinline__ unsigned swap_halves_u32(unsigned x, bool upper)
+{
+    const auto s = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return upper ? s[0] : s[1];
+}
+__device__ __forceinline__ double swap_halves(double x)
+{
+    const bool upper = (threadIdx.x & 32) != 0;
+    const unsigned hi = swap_halves_u32((unsigned)__double2hiint(x), upper);
+    const unsigned lo = swap_halves_u32((unsigned)__double2loint(x), upper);
+    return __hiloint2double((int)hi, (int)lo);
+}
+__device__ __forceinline__ float swap_halves(float x) { return __shfl_xor(x, 32, 64); }
+
 // LDS position of element i of a line: one 16-byte pad after every 8 elements, so
 // the stride-8 and stride-64 accesses of the radix-8 stages spread over all banks.
 __host__ __device__ constexpr int pidx(int i) { return i + (i >> 3); }
