@@ -13,7 +13,8 @@
 //
 // Device layouts (all FP64 complex = 16 B):
 //   om, fn : [kx][iz]   column-contiguous (H columns of N)          -- state
-//   V      : [f][kx][z] 4 fields (u, w, d_x omega, d_z omega), column-contiguous -- z-inverse outputs
+//   V      : [f][z/2][kx][z%2] 4 fields (u, w, d_x omega, d_z omega): pairs of rows
+//            interleaved (vrow), 32-byte runs of (z, z+1) per mode -- z-inverse outputs
 //   G      : [z][kx]    M x H                                     -- x-forward of u.grad(omega)
 //   tw     : W_M^t = e^{-2 pi i t / M}, t in [0, M)
 //
@@ -85,6 +86,31 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
     return (n % 8 == 0) ? (b % 8) * (n / 8) + b / 8 : b;
 }
 
+// V layout: groups of P = vp<TC>(H) consecutive rows interleaved, [f][z / P][kx][z mod P]:
+// row z of a field starts at vrow(z) and holds mode kx at + P kx.  The column kernel writes
+// each mode's P rows of a group as one run of P elements; a row reads its modes P elements
+// apart, so the 64 modes of one load instruction lie in 64 P elements (with the modes M apart,
+// as in a column-major V, they lay 6 MB apart at N = 4096, each in its own line and page),
+// and the other rows of the group, run by neighbouring workgroups of the same XCD, use the
+// rest of each line.  Timed at N = 4096 (step ms, column-major V 1.20 FP64 / 0.662 FP32):
+// runs of 32 / 64 / 128 bytes 1.21 / 1.14 / 1.16 ms FP64, 16 / 32 / 64 / 128 bytes
+// 0.76 / 0.683 / 0.645 / 0.633 ms FP32; the FP64 ensemble sizes (H <= 512) keep 128-byte
+// runs (the ensemble step with 64: 1.21 -> 1.23 ms).
+template <class TC>
+__host__ __device__ constexpr int vp(int H)
+{
+    int p = (sizeof(TC) == 16 && H >= 1024 ? 64 : 128) / (int)sizeof(TC);
+    while ((3 * H) % p != 0) p /= 2;  // whole groups: P divides M (H = 8 in FP32: P = 8)
+    return p;
+}
+template <class TC>
+__host__ __device__ constexpr size_t vrow(int z, int H)
+{
+    return (size_t)(z / vp<TC>(H)) * (vp<TC>(H) * (size_t)H) + (z % vp<TC>(H));
+}
+// the M = 3H rows group exactly
+#define DNS_V_PAIRS(H_) static_assert((3 * (H_)) % vp<cd>(H_) == 0 && (3 * (H_)) % vp<dnsfft::cf>(H_) == 0, "V row groups")
+
 __device__ __forceinline__ double inv_k2_of(double ax, double gz) { return 1.0 / (ax * ax + gz * gz + 1.0e-30); }
 __device__ __forceinline__ double inv_g_of(int iz, double gz) { return (iz >= 1 && gz != 0.0) ? 1.0 / gz : 0.0; }
 
@@ -140,6 +166,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
                                            const TC *__restrict__ twT, const cd *__restrict__ tw, const ColArgs a,
                                            BatchArgs bt, const TC *__restrict__ twH)
 {
+    DNS_V_PAIRS(H);
     constexpr int N = 2 * H;
     constexpr int M = 3 * H;
     constexpr int PJ = C * H / T;
@@ -350,11 +377,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
         }
         __syncthreads();
         lines(std::true_type{});
-        TC *out = V + (size_t)fi * a.vstride + (size_t)kx0 * M;
+        TC *out = V + (size_t)fi * a.vstride;
         for (int e = tid; e < C * M; e += T) {
             const int c = e / M, zz = e - c * M;
             const int z1 = zz / 3, z2 = zz - 3 * z1;
-            out[(size_t)c * M + zz] = lds[(z2 * C + c) * LS + pidx(z1)];
+            out[vrow<TC>(zz, H) + vp<TC>(H) * (size_t)(kx0 + c)] = lds[(z2 * C + c) * LS + pidx(z1)];
         }
         __syncthreads();
     }
@@ -371,6 +398,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1))) void k_c
     cd *__restrict__ om, cd *__restrict__ fn, const cd *__restrict__ G, cd *__restrict__ V, const cd *__restrict__ twT,
     const cd *__restrict__ tw, const ColArgs a, const cd *__restrict__ twH)
 {
+    DNS_V_PAIRS(H);
     using TC = cd;
     constexpr int N = 2 * H;
     constexpr int M = 3 * H;
@@ -504,10 +532,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1))) void k_c
             }
             __syncthreads();
             lines(std::true_type{});
-            TC *out = V + (size_t)fi * a.vstride + (size_t)kx0 * M;
+            TC *out = V + (size_t)fi * a.vstride + vp<TC>(H) * (size_t)kx0;
             for (int e = tid; e < M; e += T) {
                 const int z1 = e / 3, z2 = e - 3 * z1;
-                out[e] = lds[z2 * LS + pidx(z1)];
+                out[vrow<TC>(e, H)] = lds[z2 * LS + pidx(z1)];
             }
             __syncthreads();
         }
@@ -534,6 +562,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
                                            const TC *__restrict__ tw, double *__restrict__ cflrow,
                                            double DX, double DZ, BatchArgs bt, const TC *__restrict__ twH)
 {
+    DNS_V_PAIRS(H);
     {
         const size_t b = blockIdx.y;
         V += b * bt.s_v;
@@ -555,7 +584,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
     // rows in XCD-aware order: neighbouring rows share the 128-byte lines of the
     // column-major V, so they should meet in one L2
     const int z = xcd_tile(blockIdx.x, gridDim.x);
-    const TC *row = V + z;  // element (f, k) at f * vstride + k * M
+    const TC *row = V + vrow<TC>(z, H);  // element (f, k) at f * vstride + vp k
     // W_3^{2 n2} for n2 = 0, 1, 2 (indices (2 n2 H) mod M)
     const TC w3_1 = tw[2 * H], w3_2 = tw[H];
     const TC zero{RT(0), RT(0)};
@@ -590,8 +619,8 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             const int k = lane < 32 ? q : (q == 0 ? H / 2 : H - q);
             TC va = zero, vb = zero;  // a branch, not a select (a select of the two sources
             if (q < H / 2) {          // becomes a flat load from a scratch copy of zero)
-                va = fa[(size_t)k * M];
-                vb = fb[(size_t)k * M];
+                va = fa[vp<TC>(H) * (size_t)k];
+                vb = fb[vp<TC>(H) * (size_t)k];
             }
             a[j] = va;
             b[j] = vb;
@@ -717,6 +746,7 @@ template <class TC, int H, int T, int RPW>
 __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t vstride, TC *__restrict__ G,
                                                 const TC *__restrict__ tw, const TC *__restrict__ twH)
 {
+    DNS_V_PAIRS(H);
     using RT = decltype(TC::x);
     constexpr int LS = padded_len(H);
     constexpr int PJ = H / T;  // input pairs per lane
@@ -753,14 +783,14 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
             const int k = lane < 32 ? q : (q == 0 ? H / 2 : H - q);
             TC va = zero, vb = zero;
             if (q < H / 2) {
-                va = fa[(size_t)k * M];
-                vb = fb[(size_t)k * M];
+                va = fa[vp<TC>(H) * (size_t)k];
+                vb = fb[vp<TC>(H) * (size_t)k];
             }
             a[j] = va;
             b[j] = vb;
         }
     };
-    issue(V + row_of(0), ra0, rb0);
+    issue(V + vrow<TC>(row_of(0), H), ra0, rb0);
 #pragma unroll 1
     for (int i = 0; i < RPW; ++i) {
         // the thread index through an empty asm per row: the index arithmetic of the row
@@ -768,7 +798,7 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
         tid = opaque((int)threadIdx.x);
         lane = tid & 63;
         const int z = row_of(i);
-        const TC *row = V + z;
+        const TC *row = V + vrow<TC>(z, H);
 #pragma unroll
         for (int ph = 0; ph < 2; ++ph) {
 #pragma unroll
@@ -822,7 +852,7 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
             }
             // the next row's first pair, into the registers this row's pairs have freed: in
             // flight during the forward transform
-            if (ph == 1 && i + 1 < RPW) issue(V + row_of(i + 1), ra0, rb0);
+            if (ph == 1 && i + 1 < RPW) issue(V + vrow<TC>(row_of(i + 1), H), ra0, rb0);
             __syncthreads();
         }
         TC *A = lds;
@@ -855,6 +885,7 @@ template <class TC, int H, int T, int R, int MINW, int PLAN = 0>
 __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, size_t vstride, TC *__restrict__ G,
                                               const TC *__restrict__ tw, BatchArgs bt)
 {
+    DNS_V_PAIRS(H);
     {
         const size_t b = blockIdx.y;
         V += b * bt.s_v;
@@ -873,7 +904,7 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
     const int lane = tid & 63;
     const int z0 = xcd_tile(blockIdx.x, gridDim.x) * R;
     const int r = (lane & 31) % R, sq = (lane & 31) / R;
-    const TC *row = V + z0 + r;  // element (f, k) of row z0 + r at f * vstride + k * M
+    const TC *row = V + vrow<TC>(z0 + r, H);  // element (f, k) of row z0 + r at f * vstride + vp k
     const TC w3_1 = tw[2 * H], w3_2 = tw[H];
     const TC zero{RT(0), RT(0)};
     const TC wm1 = tw[1], wm2 = tw[2];
@@ -898,8 +929,8 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
         for (int j = 0; j < PJ; ++j) {
             int q;
             const int k = mode_of(j, &q);
-            a[j] = fa[(size_t)k * M];
-            b[j] = fb[(size_t)k * M];
+            a[j] = fa[vp<TC>(H) * (size_t)k];
+            b[j] = fb[vp<TC>(H) * (size_t)k];
         }
     };
     // LDS line of residue n2 of row r: residue-major, all rows in one transform call
@@ -981,24 +1012,25 @@ __global__ __launch_bounds__(T) void k_row_phys(const cd *__restrict__ Va, const
                                                 const cd *__restrict__ tw, double *__restrict__ out0,
                                                 double *__restrict__ out1, int out_mode)
 {
+    DNS_V_PAIRS(H);
     constexpr int M = 3 * H;
     constexpr int PJ = (H + T - 1) / T;
     constexpr int LS = padded_len(H);
     __shared__ cd A[LS];
     const int tid = threadIdx.x;
     const int z = blockIdx.x;
-    const cd *a = Va + z;  // column-major: element k at k * M
-    const cd *b = Vb ? Vb + z : nullptr;
+    const cd *a = Va + vrow<cd>(z, H);  // element k at vp k
+    const cd *b = Vb ? Vb + vrow<cd>(z, H) : nullptr;
     for (int x2 = 0; x2 < 3; ++x2) {
         const cd w3 = tw[(2 * x2 * H) % M];
         for (int k1 = tid; k1 < H; k1 += T) {
-            const cd av = a[(size_t)k1 * M];
-            const cd bv = b ? b[(size_t)k1 * M] : mk(0.0, 0.0);
+            const cd av = a[vp<cd>(H) * (size_t)k1];
+            const cd bv = b ? b[vp<cd>(H) * (size_t)k1] : mk(0.0, 0.0);
             cd zlo = (k1 == 0) ? mk(av.x, bv.x) : mk(av.x - bv.y, av.y + bv.x);
             cd zhi = mk(0.0, 0.0);
             if (k1 > 0) {
-                const cd a2 = a[(size_t)(H - k1) * M];
-                const cd b2 = b ? b[(size_t)(H - k1) * M] : mk(0.0, 0.0);
+                const cd a2 = a[vp<cd>(H) * (size_t)(H - k1)];
+                const cd b2 = b ? b[vp<cd>(H) * (size_t)(H - k1)] : mk(0.0, 0.0);
                 zhi = mk(a2.x + b2.y, b2.x - a2.y);
             }
             A[pidx(k1)] = mulc(add(zlo, mulc(zhi, w3)), tw[opaque(x2 * k1)]);
