@@ -13,9 +13,9 @@
 //
 // Device layouts (all FP64 complex = 16 B):
 //   om, fn : [kx][iz]   column-contiguous (H columns of N)          -- state
-//   V      : [f][z/2][kx][z%2] 4 fields (u, w, d_x omega, d_z omega): pairs of rows
-//            interleaved (vrow), 32-byte runs of (z, z+1) per mode -- z-inverse outputs
-//   G      : [z][kx]    M x H                                     -- x-forward of u.grad(omega)
+//   V      : [f][z/P][kx][z%P] 4 fields (u, w, d_x omega, d_z omega), groups of P rows
+//            (vrow)                                             -- z-inverse outputs
+//   G      : [kx/Q][z][kx%Q] M x H, groups of Q modes (gpos)        -- x-forward of u.grad(omega)
 //   tw     : W_M^t = e^{-2 pi i t / M}, t in [0, M)
 //
 // Per step two kernels run (reference step order mem_cuda.cu:1851-1857):
@@ -107,6 +107,24 @@ template <class TC>
 __host__ __device__ constexpr size_t vrow(int z, int H)
 {
     return (size_t)(z / vp<TC>(H)) * (vp<TC>(H) * (size_t)H) + (z % vp<TC>(H));
+}
+// G layout, the same idea across the columns: groups of Q = gq<TC>(H) consecutive modes,
+// [kx / Q][z][kx mod Q]; a row writes runs of Q modes, a column reads its rows Q elements
+// apart (with a row-major G, H elements apart).  Runs of 64 bytes timed best at N = 4096
+// (row-major G -> 64-byte runs: step 1.148 -> 1.118 ms FP64, column kernel 0.567 -> 0.535;
+// 0.640 -> 0.633 ms FP32; 128-byte runs: FP64 1.17 ms); the FP64 ensemble sizes keep
+// 128-byte runs.
+template <class TC>
+__host__ __device__ constexpr int gq(int H)
+{
+    int q = (sizeof(TC) == 16 && H < 1024 ? 128 : 64) / (int)sizeof(TC);
+    while (H % q != 0) q /= 2;  // whole groups: Q divides H
+    return q;
+}
+template <class TC>
+__host__ __device__ constexpr size_t gpos(int z, int kx, int H)
+{
+    return (size_t)(kx / gq<TC>(H)) * (3 * (size_t)H * gq<TC>(H)) + (size_t)z * gq<TC>(H) + (kx % gq<TC>(H));
 }
 // the M = 3H rows group exactly
 #define DNS_V_PAIRS(H_) static_assert((3 * (H_)) % vp<cd>(H_) == 0 && (3 * (H_)) % vp<dnsfft::cf>(H_) == 0, "V row groups")
@@ -252,7 +270,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
         for (int e = tid; e < 3 * C * H; e += T) {
             const int c = e % C, q = e / C;
             const int p = q / 3, z2 = q - 3 * p;
-            lds[(z2 * C + c) * LS + pidx(p)] = G[(size_t)(3 * zpos(p) + z2) * H + kx0 + c];
+            lds[(z2 * C + c) * LS + pidx(p)] = G[gpos<TC>(3 * zpos(p) + z2, kx0 + c, H)];
         }
         // om, fn of the owned modes (lo: iz = k1, hi: iz = k1 + H), in flight during the
         // forward transforms when the twiddles come from LDS
@@ -436,7 +454,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1))) void k_c
     cd gx[GJ];
 #pragma unroll
     for (int r = 0; r < GJ; ++r) {
-        const cd v = G[(size_t)grow(threadIdx.x + r * T) * H + col_of(0)];
+        const cd v = G[gpos<cd>(grow(threadIdx.x + r * T), col_of(0), H)];
         gx[r] = mk(v.x, v.y);
     }
 #pragma unroll 1
@@ -526,7 +544,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1))) void k_c
                 const int kn = col_of(i + 1);
 #pragma unroll
                 for (int r = 0; r < GJ; ++r) {
-                    const cd v = G[(size_t)grow(tid + r * T) * H + kn];
+                    const cd v = G[gpos<cd>(grow(tid + r * T), kn, H)];
                     gx[r] = mk(v.x, v.y);
                 }
             }
@@ -702,7 +720,6 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
         TC *A = lds;
         TC *B = lds + LS;
         fft_lines_from_pos<H, LS, 2, T, false, (PLAN & 2) ? 16 : 8, opq_row<TC>(H) || (PLAN & 1) != 0, ip_row<TC>(H)>(lds, twl, 1);
-        TC *g = G + (size_t)z * H;
         const RT half = RT(0.5);
         for (int k = tid; k < H; k += T) {
             const TC za = A[pidx(k)];
@@ -711,7 +728,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             const TC F0{half * s.x, half * s.y};
             const TC F1{half * d.y, -half * d.x};  // d / (2 i)
             const TC F2 = B[pidx(k)];
-            g[k] = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
+            G[gpos<TC>(z, k, H)] = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
         }
     }
 
@@ -858,7 +875,6 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
         TC *A = lds;
         TC *B = lds + LS;
         fft_lines_from_pos<H, LS, 2, T, false, 8, opq_row<TC>(H)>(lds, twl, 1);
-        TC *g = G + (size_t)z * H;
         const RT half = RT(0.5);
         for (int k = tid; k < H; k += T) {
             const TC za = A[pidx(k)];
@@ -867,7 +883,7 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
             const TC F0{half * s.x, half * s.y};
             const TC F1{half * d.y, -half * d.x};
             const TC F2 = B[pidx(k)];
-            g[k] = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
+            G[gpos<TC>(z, k, H)] = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
         }
         __syncthreads();  // the next row writes the lines this loop reads
     }
@@ -999,7 +1015,7 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
         const TC F0{half * s.x, half * s.y};
         const TC F1{half * d.y, -half * d.x};  // d / (2 i)
         const TC F2 = lds[line(rr, 1) * LS + pidx(k)];
-        G[(size_t)(z0 + rr) * H + k] = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
+        G[gpos<TC>(z0 + rr, k, H)] = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
     }
 }
 
