@@ -4,10 +4,11 @@
 // Layouts (elements of the transform precision unless noted; hs, ms: padded row strides
 // of the H-wide and M-wide rows):
 //   om, fn  [iz][kx] FP64, N x H (unpadded: the host layout)
-//   Y, Z    [f][z][kx], 2 or 4 fields, M rows of stride hs
+//   Y       [f][kx/BZ][z][kx%BZ], 2 or 4 fields of M x hs: column tiles (z passes only)
+//   Z       [f][z][kx], 2 or 4 fields, M rows of stride hs
 //   X       [p][z][x1], 1 or 2 planes, M rows of stride ms
 //   Yq, Qh  [q][.], M/2 rows of stride ms
-//   W       [z1][kx], M rows of stride hs
+//   W       [kx/BZ][z][kx%BZ], M x hs: column tiles (z passes only)
 #pragma once
 
 #include <hip/hip_runtime_api.h>

@@ -125,6 +125,18 @@ hipError_t with_big_m(int m, F &&f)
 template <class TC>
 constexpr int bz_of() { return sizeof(TC) == 16 ? 8 : 16; }
 
+// Y (pass 1 -> 2) and W (pass 6 -> 7), which only the z-direction passes touch, are stored
+// in column tiles of BZ modes, [kx / BZ][z][kx mod BZ] (each field M x hs elements as before):
+// a z-pass workgroup (one tile, all rows) then stays within M x 128 bytes (5.6 MB at
+// N = 29400, a few pages), where rows of stride hs (235 KB) spread its 210 rows over 49 MB
+// and a third of its address translations missed the L1 TLB.
+template <class TC>
+__device__ __forceinline__ size_t ztile(int z, int kx, int M)
+{
+    constexpr int BZ = bz_of<TC>();
+    return (size_t)(kx / BZ) * ((size_t)M * BZ) + (size_t)z * BZ + (kx % BZ);
+}
+
 // spectrum multiplier of field f on omega-hat (the fused column kernel's conventions:
 // the multipliers listed with DNS_F_*)
 __device__ __forceinline__ cd spec_of(int f, cd om, int ix, int iz, int N)
@@ -250,7 +262,7 @@ __global__ __launch_bounds__(T) void k_zinv_a(const cd *__restrict__ om, TC *__r
         const int kx = kx0 + c;
         if (kx < H) {
             const TC w = twid2<TC, L1, true>(w2, wb, n2 * k1);
-            const size_t o = (size_t)(k1 * L2 + n2) * hs + kx;
+            const size_t o = ztile<TC>(k1 * L2 + n2, kx, M_);
             ya[o] = mul(lds[c * LS + pidx(k1)], w);
             yb[o] = mul(lds[(BZ + c) * LS + pidx(k1)], w);
         }
@@ -279,7 +291,7 @@ __global__ __launch_bounds__(T) void k_zinv_b(const TC *__restrict__ Y, TC *__re
         const int e = threadIdx.x + i * T;
         const int c = e % BZ, n2 = e / BZ;
         const int kx = kx0 + c;
-        const size_t o = (size_t)(k1 * L2 + n2) * hs + kx;
+        const size_t o = ztile<TC>(k1 * L2 + n2, kx, M_);
         const bool ok = e < CNT && kx < H;
         va[i] = ok ? ya[o] : TC{0, 0};
         vb[i] = ok ? yb[o] : TC{0, 0};
@@ -583,7 +595,7 @@ __global__ __launch_bounds__(T) void k_zfwd_a(const TC *__restrict__ Qh, TC *__r
         const int c = e % BZ, rest = e / BZ, k1 = rest % L1, r = rest / L1;
         const int kx = kx0 + c, n2 = 2 * j + r;
         if (kx < H)
-            W[(size_t)(k1 * L2 + n2) * hs + kx] = mul(lds[(r * BZ + c) * LS + pidx(k1)], twid2<TC, L1, false>(w2, wb, n2 * k1));
+            W[ztile<TC>(k1 * L2 + n2, kx, M_)] = mul(lds[(r * BZ + c) * LS + pidx(k1)], twid2<TC, L1, false>(w2, wb, n2 * k1));
     }
 }
 
@@ -609,7 +621,7 @@ __global__ __launch_bounds__(T) void k_zfwd_b(const TC *__restrict__ W, cd *__re
         const int e = threadIdx.x + i * T;
         const int c = e % BZ, n2 = e / BZ;
         const int kx = kx0 + c;
-        wv[i] = (e < CNT && kx < H) ? W[(size_t)(k1 * L2 + n2) * hs + kx] : TC{0, 0};
+        wv[i] = (e < CNT && kx < H) ? W[ztile<TC>(k1 * L2 + n2, kx, M_)] : TC{0, 0};
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
