@@ -548,13 +548,27 @@ dns_status compute_cflm(dns_handle *h, double *cflm)
     return DNS_OK;
 }
 
-// host [iz][ix] <-> device [ix][iz] through the diagnostics buffer as staging
+// elements of om (and of fn) on the device
+size_t state_elems(const dns_handle *h)
+{
+    return h->big ? (size_t)h->N * 8 * dns_state_tiles(h->H) : (size_t)h->N * h->H;
+}
+
+// host [iz][ix] <-> device [ix][iz] through the diagnostics buffer as staging (fused path);
+// host rows <-> the state's column tiles through buffer B (multi-pass path; B holds nothing
+// that outlives a step)
 dns_status upload_spec(dns_handle *h, const double *host, cd *dst)
 {
     const size_t n = (size_t)h->N * h->H;
     h->x0_valid = false;  // the state changes
-    if (h->big) {  // the multi-pass state is in the host layout
-        HIPCHK(h, hipMemcpyAsync(dst, host, sizeof(cd) * n, hipMemcpyHostToDevice, h->stream));
+    if (h->big) {
+        HIPCHK(h, hipMemcpyAsync(h->BB, host, sizeof(cd) * n, hipMemcpyHostToDevice, h->stream));
+        BigReq r{};
+        r.what = BIG_STATE_TILE;
+        r.a = h->BB;
+        r.b = dst;
+        dns_status st = big_launch(h, -1, r);
+        if (st != DNS_OK) return st;
         HIPCHK(h, hipStreamSynchronize(h->stream));
         return DNS_OK;
     }
@@ -570,7 +584,13 @@ dns_status download_spec(dns_handle *h, const cd *src, double *host)
 {
     const size_t n = (size_t)h->N * h->H;
     if (h->big) {
-        HIPCHK(h, hipMemcpyAsync(host, src, sizeof(cd) * n, hipMemcpyDeviceToHost, h->stream));
+        BigReq r{};
+        r.what = BIG_STATE_UNTILE;
+        r.a = src;
+        r.b = h->BB;
+        dns_status st = big_launch(h, -1, r);
+        if (st != DNS_OK) return st;
+        HIPCHK(h, hipMemcpyAsync(host, h->BB, sizeof(cd) * n, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         return DNS_OK;
     }
@@ -697,7 +717,8 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
         return bail(fail(h, DNS_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
 
-    const size_t nspec = (size_t)h->N * h->H;
+    // the multi-pass state is in column tiles of 8 modes (the last one padded)
+    const size_t nspec = state_elems(h);
     const size_t nmid = (size_t)h->M * h->H;
     dns_status st;
     if ((st = dev_alloc(h, (void **)&h->om, sizeof(cd) * nspec, "om2")) != DNS_OK) return bail(st);
@@ -812,7 +833,7 @@ extern "C" dns_status dns_init_ic(dns_handle *h, int kind, uint64_t seed, char *
     h->sc.visc = visc;
     dns_status st = upload_spec(h, om2.data(), h->om);
     if (st != DNS_OK) return st;
-    HIPCHK(h, hipMemsetAsync(h->fn, 0, sizeof(cd) * (size_t)h->N * h->H, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->fn, 0, sizeof(cd) * state_elems(h), h->stream));
     h->have_ic = true;
     return refresh_v(h);
 }
@@ -927,8 +948,8 @@ extern "C" dns_status dns_energy_enstrophy(dns_handle *h, double *E, double *Z)
 {
     if (!h || !E || !Z) return DNS_E_ARG;
     HIPCHK(h, hipSetDevice(h->device));
-    // per-column partials (fused layout [kx][iz]) or per-row partials (multi-pass [iz][kx])
-    const int parts = h->big ? h->N : h->H;
+    // per-column partials (fused layout [kx][iz]) or per-tile partials (multi-pass column tiles)
+    const int parts = h->big ? dns_state_tiles(h->H) : h->H;
     if (h->big) {
         BigReq r{};
         r.what = BIG_ENERGY;
@@ -959,7 +980,7 @@ extern "C" dns_status dns_pal_over_ens_kmax2(dns_handle *h, double *out)
 {
     if (!h || !out) return DNS_E_ARG;
     HIPCHK(h, hipSetDevice(h->device));
-    const int parts = h->big ? h->N : h->H;
+    const int parts = h->big ? dns_state_tiles(h->H) : h->H;
     HIPCHK(h, dns_launch_palin(h->om, h->N, h->big ? 1 : 0, h->partEZ, h->partEZ + parts, h->stream));
     std::vector<double> p(2 * (size_t)parts);
     HIPCHK(h, hipMemcpyAsync(p.data(), h->partEZ, sizeof(double) * p.size(), hipMemcpyDeviceToHost, h->stream));

@@ -3,7 +3,8 @@
 //
 // Layouts (elements of the transform precision unless noted; hs, ms: padded row strides
 // of the H-wide and M-wide rows):
-//   om, fn  [iz][kx] FP64, N x H (unpadded: the host layout)
+//   om, fn  [kx/8][iz][kx%8] FP64, column tiles of 8 modes (dns_state_pos; N x 8 tiles
+//           elements, the last tile padded): a z pass touches one tile's N x 128 bytes
 //   Y       [f][kx/BZ][z][kx%BZ], 2 or 4 fields of M x hs: column tiles (z passes only)
 //   Z       [f][z][kx], 2 or 4 fields, M rows of stride hs
 //   X       [p][z][x1], 1 or 2 planes, M rows of stride ms
@@ -30,8 +31,17 @@ enum {
     BIG_LINE_INV = 8,  // FP64: `lines` natural-order inverse line FFTs a -> b (scratch)
     BIG_ROWS = 9,      // FP64: Z fields 0 (+ i 1 when planes == 2) -> rows R [z][x]  a = Z, b = R
     BIG_ROWS_PHYS = 10,  // FP64: rows a -> out0 (Re, or |.| with out_mode 1)
-    BIG_ENERGY = 11    // om -> per-row E, Z partials out0[N], out1[N]
+    BIG_ENERGY = 11,   // om -> per-tile E, Z partials out0[tiles], out1[tiles]
+    BIG_STATE_TILE = 12,   // host-layout rows a [iz][kx] (N x H) -> state tiles b
+    BIG_STATE_UNTILE = 13  // state tiles a -> host-layout rows b
 };
+
+// the state's column tiles: element (iz, kx) of om, fn on the multi-pass path
+__host__ __device__ inline size_t dns_state_pos(int iz, int kx, int N)
+{
+    return (size_t)(kx >> 3) * ((size_t)N * 8) + (size_t)iz * 8 + (kx & 7);
+}
+inline int dns_state_tiles(int H) { return (H + 7) / 8; }
 
 struct BigFields {
     int f[4];   // DNS_F_* of each field (pairs (f0, f1), (f2, f3))

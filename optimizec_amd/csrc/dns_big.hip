@@ -130,6 +130,8 @@ constexpr int bz_of() { return sizeof(TC) == 16 ? 8 : 16; }
 // a z-pass workgroup (one tile, all rows) then stays within M x 128 bytes (5.6 MB at
 // N = 29400, a few pages), where rows of stride hs (235 KB) spread its 210 rows over 49 MB
 // and a third of its address translations missed the L1 TLB.
+// The state om, fn is stored the same way on this path (dns_state_pos, dns_big.h): passes 1
+// and 7 then read and write it within one tile's N x 128 bytes.
 template <class TC>
 __device__ __forceinline__ size_t ztile(int z, int kx, int M)
 {
@@ -233,7 +235,7 @@ __global__ __launch_bounds__(T) void k_zinv_a(const cd *__restrict__ om, TC *__r
         const int c = e % BZ, n1 = e / BZ;
         const int kx = kx0 + c, zp = n1 * L2 + n2;
         const int iz = zp < H ? zp : (zp >= 2 * H ? zp - H : -1);
-        ov[i] = (e < CNT && kx < H && iz >= 0) ? om[(size_t)iz * H + kx] : cd{0.0, 0.0};
+        ov[i] = (e < CNT && kx < H && iz >= 0) ? om[dns_state_pos(iz, kx, N)] : cd{0.0, 0.0};
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
@@ -642,7 +644,7 @@ __global__ __launch_bounds__(T) void k_zfwd_b(const TC *__restrict__ W, cd *__re
         const double ax = double(kx), gz = gam(iz, N);
         const double ARG = a.VT * (ax * ax + gz * gz);
         const double DEN = 1.0 + ARG, c1 = 1.0 - ARG;
-        const size_t io = (size_t)iz * H + kx;
+        const size_t io = dns_state_pos(iz, kx, N);
         const cd o = om[io], f = fn[io];
         const cd num{c1 * o.x + a.c2 * FN.x + a.c3 * f.x, c1 * o.y + a.c2 * FN.y + a.c3 * f.y};
         om[io] = cd{num.x / DEN, num.y / DEN};
@@ -724,16 +726,20 @@ __global__ void k_rows_phys(const cd *__restrict__ R, size_t n, double *__restri
     }
 }
 
-// E, Z partial sums per row iz of the [iz][kx] state (the energy convention of k_energy)
-__global__ __launch_bounds__(256) void k_energy_rows(const cd *__restrict__ om, int N, double *__restrict__ pe,
-                                                     double *__restrict__ pz)
+// E, Z partial sums per column tile of the state (the energy convention of k_energy): block
+// b sums the modes kx = 8 b .. 8 b + 7 over all iz (one tile's contiguous N x 128 bytes)
+__global__ __launch_bounds__(256) void k_energy_tiles(const cd *__restrict__ om, int N, double *__restrict__ pe,
+                                                      double *__restrict__ pz)
 {
     __shared__ double re[4], rz[4];
-    const int H = N / 2, iz = blockIdx.x;
-    const double gz = gam(iz, N);
+    const int H = N / 2, b = blockIdx.x;
+    const cd *t = om + dns_state_pos(0, 8 * b, N);
     double e = 0.0, zs = 0.0;
-    for (int kx = threadIdx.x; kx < H; kx += 256) {
-        const cd o = om[(size_t)iz * H + kx];
+    for (int i = threadIdx.x; i < 8 * N; i += 256) {
+        const int iz = i >> 3, kx = 8 * b + (i & 7);
+        if (kx >= H) continue;
+        const cd o = t[i];
+        const double gz = gam(iz, N);
         const double a2 = o.x * o.x + o.y * o.y, m = kx == 0 ? 1.0 : 2.0;
         const double k2 = double(kx) * kx + gz * gz;
         zs += m * a2;
@@ -749,8 +755,24 @@ __global__ __launch_bounds__(256) void k_energy_rows(const cd *__restrict__ om, 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        pe[iz] = re[0] + re[1] + re[2] + re[3];
-        pz[iz] = rz[0] + rz[1] + rz[2] + rz[3];
+        pe[b] = re[0] + re[1] + re[2] + re[3];
+        pz[b] = rz[0] + rz[1] + rz[2] + rz[3];
+    }
+}
+
+// the state between the host layout [iz][kx] (N x H) and the tiles: to_tiles 1 reads rows
+// from src and writes tiles to dst, 0 the other way
+__global__ void k_state_tile(const cd *__restrict__ src, cd *__restrict__ dst, int N, int to_tiles)
+{
+    const int H = N / 2;
+    const size_t total = (size_t)N * H;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(e / H), kx = (int)(e % H);
+        const size_t t = dns_state_pos(iz, kx, N);
+        if (to_tiles)
+            dst[t] = src[e];
+        else
+            dst[e] = src[t];
     }
 }
 
@@ -882,7 +904,14 @@ hipError_t dns_big_launch(const BigReq &r)
                                static_cast<const cd *>(r.a), (size_t)M * M, r.out0, r.out_mode);
             return hipGetLastError();
         case BIG_ENERGY:
-            hipLaunchKernelGGL(k_energy_rows, dim3(2 * r.H), dim3(256), 0, r.stream, r.om, 2 * r.H, r.out0, r.out1);
+            hipLaunchKernelGGL(k_energy_tiles, dim3(dns_state_tiles(r.H)), dim3(256), 0, r.stream, r.om, 2 * r.H, r.out0,
+                               r.out1);
+            return hipGetLastError();
+        case BIG_STATE_TILE:
+        case BIG_STATE_UNTILE:
+            hipLaunchKernelGGL(k_state_tile, dim3(grid_for((size_t)2 * r.H * r.H)), dim3(256), 0, r.stream,
+                               static_cast<const cd *>(r.a), static_cast<cd *>(r.b), 2 * r.H,
+                               r.what == BIG_STATE_TILE ? 1 : 0);
             return hipGetLastError();
         default:
             if (!r.fp32) return launch_t<cd>(r, M);

@@ -80,8 +80,8 @@ int dns_kernels_col_tile(int H);
 hipError_t dns_kernels_launch(int H, const LaunchReq &r);
 hipError_t dns_launch_energy(const dnsfft::cd *om, int N, int H, double *partE, double *partZ, hipStream_t s);
 // ADAPT_VISC input partials (one per line of the state: N / 2 columns of the fused [kx][iz]
-// layout, or N rows of the multi-pass [iz][kx] layout with rowmajor = 1)
-hipError_t dns_launch_palin(const dnsfft::cd *om, int N, int rowmajor, double *partT, double *partP, hipStream_t s);
+// layout, or, with tiled = 1, one per column tile of 8 modes of the multi-pass layout)
+hipError_t dns_launch_palin(const dnsfft::cd *om, int N, int tiled, double *partT, double *partP, hipStream_t s);
 hipError_t dns_launch_minmax(const double *f, size_t n, double *out, int blocks, hipStream_t s);
 hipError_t dns_launch_pixels(const double *f, size_t n, double mn, double rng, int is_const,
                              unsigned long long *out, int blocks, hipStream_t s);

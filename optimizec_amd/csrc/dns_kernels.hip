@@ -1117,15 +1117,18 @@ __global__ __launch_bounds__(256) void k_energy(const cd *__restrict__ om, int N
 // (0, 0) mode is left out of the enstrophy sum.  Block b sums line b of the state:
 // column kx = b of [kx][iz] (rowmajor 0, fused layout) or row iz = b of [iz][kx]
 // (rowmajor 1, multi-pass layout).
-__global__ __launch_bounds__(256) void k_palin(const cd *__restrict__ om, int N, int rowmajor,
+__global__ __launch_bounds__(256) void k_palin(const cd *__restrict__ om, int N, int tiled,
                                                double *__restrict__ partT, double *__restrict__ partP)
 {
     __shared__ double red[4];
     const int H = N / 2, b = blockIdx.x;
-    const int len = rowmajor ? H : N;
+    // fused layout [kx][iz]: block b = column kx; multi-pass column tiles [kx/8][iz][kx%8]:
+    // block b = tile of the modes 8 b .. 8 b + 7 (contiguous, N x 8 elements)
+    const int len = tiled ? 8 * N : N;
     double tot = 0.0, pal = 0.0;
     for (int i = threadIdx.x; i < len; i += 256) {
-        const int kx = rowmajor ? i : b, iz = rowmajor ? b : i;
+        const int kx = tiled ? 8 * b + (i & 7) : b, iz = tiled ? (i >> 3) : i;
+        if (kx >= H) continue;
         const cd o = om[(size_t)b * len + i];
         const double w = (kx == 0 || kx == H - 1) ? 1.0 : 2.0;
         const double gz = gamma_of(iz, N);
@@ -1490,9 +1493,9 @@ hipError_t dns_launch_energy(const cd *om, int N, int H, double *partE, double *
     return hipGetLastError();
 }
 
-hipError_t dns_launch_palin(const cd *om, int N, int rowmajor, double *partT, double *partP, hipStream_t s)
+hipError_t dns_launch_palin(const cd *om, int N, int tiled, double *partT, double *partP, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_palin, dim3(rowmajor ? N : N / 2), dim3(256), 0, s, om, N, rowmajor, partT, partP);
+    hipLaunchKernelGGL(k_palin, dim3(tiled ? (N / 2 + 7) / 8 : N / 2), dim3(256), 0, s, om, N, tiled, partT, partP);
     return hipGetLastError();
 }
 
