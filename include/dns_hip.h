@@ -53,8 +53,13 @@ typedef enum {
                            transforms run as a DFT-matrix product on the matrix cores: three
                            BF16 products (hi*hi + hi*lo + lo*hi), FP32 accumulation
                            (N = 256, 512, 1024, 2048, 4096; N = 29400 on the multi-pass path) */
-    DNS_PREC_FP16 = 3   /* as DNS_PREC_BF16X3 with one FP16 product per stage, each butterfly's
+    DNS_PREC_FP16 = 3,  /* as DNS_PREC_BF16X3 with one FP16 product per stage, each butterfly's
                            inputs scaled by a power of two into (-1, 1) (same sizes)          */
+    DNS_PREC_FP16_STORE = 4 /* as DNS_PREC_FP32 (vector FP32 stages), with the intermediates
+                           between the step's two kernels (the z-inverted fields and the
+                           x-forward product) stored as FP16 pairs under a power-of-two scale
+                           per stored line: half the bytes of DNS_PREC_FP32 (N = 4096, fused
+                           path)                                                              */
 } dns_precision;
 
 typedef enum {

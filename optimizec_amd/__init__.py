@@ -19,11 +19,12 @@ import numpy as np
 __all__ = ["Solver", "Scalars", "DnsError", "library_path", "load_library", "ic_pao_host",
            "ic_k4exp_host", "supported_n", "FIELD_U", "FIELD_W", "FIELD_OMEGA", "FIELD_PSI",
            "FIELD_KINETIC", "IC_PAO_KM3", "IC_K4EXP", "PREC_FP64", "PREC_FP32", "PREC_BF16X3", "PREC_FP16",
+           "PREC_FP16_STORE",
            "PATH_AUTO", "PATH_FUSED", "PATH_MULTIPASS"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 IC_PAO_KM3, IC_K4EXP = 0, 1
-PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP16 = 0, 1, 2, 3
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP16, PREC_FP16_STORE = 0, 1, 2, 3, 4
 PATH_AUTO, PATH_FUSED, PATH_MULTIPASS = 0, 1, 2
 FIELD_U, FIELD_W, FIELD_OMEGA, FIELD_PSI, FIELD_KINETIC = 0, 1, 2, 3, 4
 _STATUS = {0: "DNS_OK", 1: "DNS_E_ARG", 2: "DNS_E_UNSUPPORTED", 3: "DNS_E_HIP", 4: "DNS_E_NOMEM",
@@ -198,7 +199,9 @@ class Solver:
                  precision: int = PREC_FP64, path: int = PATH_AUTO):
         """precision: PREC_FP64, or PREC_FP32 (FP32 transforms, FP64 state/update/dt), or
         PREC_BF16X3 / PREC_FP16 (the same with the transforms' radix stages as DFT-matrix
-        products on the matrix cores: three BF16 products, or one scaled FP16 product).
+        products on the matrix cores: three BF16 products, or one scaled FP16 product), or
+        PREC_FP16_STORE (PREC_FP32 with the intermediates between the step's two kernels
+        stored as power-of-two-scaled FP16 pairs; N=4096, fused path).
         path: PATH_AUTO (fused kernels where built, else multi-pass), PATH_FUSED or
         PATH_MULTIPASS (batched four-step line FFTs through HBM)."""
         L = load_library()

@@ -78,7 +78,9 @@ struct dns_handle {
     int N = 0, M = 0, H = 0, NK = 0;
     int device = 0;
     int fp32 = 0;               // 1: V, G and the transforms in FP32 (state stays FP64)
-    int engine = 0;             // with fp32: line-transform stages 0 vector, 1 BF16x3 / 2 FP16 matrix cores
+    int engine = 0;             // with fp32: line-transform stages 0 vector, 1 BF16x3 / 2 FP16 matrix cores,
+                                // 3 vector with V, G stored as FP16 pairs
+    unsigned *sexp = nullptr;   // engine 3: the storage exponents, vexp (H) then gexp (M)
     bool big = false;           // multi-pass path (dns_big.hip) instead of the fused kernels
     void *BA = nullptr;         // multi-pass work buffers (dns_big.h layouts): Y, X, Qh and the
     void *BB = nullptr;         // diagnostics rows in BA; Z, Yq, W, diagnostics plane in BB
@@ -409,6 +411,14 @@ ColArgs inverse_args(dns_handle *h, int nf, const int *fields, size_t vstride)
     return a;
 }
 
+// the FP16 storage exponents of V (per mode) and G (per row), engine 3
+void set_storage_exponents(dns_handle *h, LaunchReq &r)
+{
+    if (h->engine != 3) return;
+    r.vexp = h->sexp;
+    r.gexp = reinterpret_cast<int *>(h->sexp + h->H);
+}
+
 // z-inverse of the step's four spectra (u, w, d_x omega, d_z omega) of the current om
 dns_status refresh_v(dns_handle *h)
 {
@@ -429,6 +439,7 @@ dns_status refresh_v(dns_handle *h)
     r.twT = h->twT;
     r.twH = h->twH;
     r.col = inverse_args(h, 4, f4, (size_t)h->M * h->H);
+    set_storage_exponents(h, r);
     dns_status st = launch(h, -1, r);
     if (st == DNS_OK) h->v_valid = true;
     return st;
@@ -670,7 +681,7 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     if (!dns_supported_path(cfg->N, cfg->path))
         return fail(nullptr, DNS_E_UNSUPPORTED, "N=%d: no kernels built for H=N/2=%d, M=3N/2=%d on path %d", cfg->N,
                     cfg->N / 2, 3 * cfg->N / 2, cfg->path);
-    if (cfg->precision < DNS_PREC_FP64 || cfg->precision > DNS_PREC_FP16)
+    if (cfg->precision < DNS_PREC_FP64 || cfg->precision > DNS_PREC_FP16_STORE)
         return fail(nullptr, DNS_E_UNSUPPORTED, "precision %d is not built", cfg->precision);
     if (cfg->precision != DNS_PREC_FP64 && cfg->N < 128)
         return fail(nullptr, DNS_E_UNSUPPORTED, "FP32 transforms are built for N >= 128");
@@ -681,6 +692,10 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
             return fail(nullptr, DNS_E_UNSUPPORTED, "matrix-core line transforms are not built for N=%d on path %d",
                         cfg->N, cfg->path);
     }
+    if (cfg->precision == DNS_PREC_FP16_STORE &&
+        (cfg->path == DNS_PATH_MULTIPASS || !dns_kernels_supported(cfg->N / 2) || !dns_kernels_h16_supported(cfg->N / 2)))
+        return fail(nullptr, DNS_E_UNSUPPORTED, "FP16 storage of the intermediates is built for N=4096 on the fused path, not N=%d path %d",
+                    cfg->N, cfg->path);
     if (!(cfg->Re > 0.0)) return fail(nullptr, DNS_E_ARG, "Re must be positive");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
@@ -695,7 +710,10 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     h->NK = 3 * cfg->N / 4 + 1;
     h->device = cfg->device;
     h->fp32 = cfg->precision != DNS_PREC_FP64 ? 1 : 0;
-    h->engine = cfg->precision == DNS_PREC_BF16X3 ? 1 : cfg->precision == DNS_PREC_FP16 ? 2 : 0;
+    h->engine = cfg->precision == DNS_PREC_BF16X3       ? 1
+                : cfg->precision == DNS_PREC_FP16       ? 2
+                : cfg->precision == DNS_PREC_FP16_STORE ? 3
+                                                        : 0;
     h->big = cfg->path == DNS_PATH_MULTIPASS || (cfg->path == DNS_PATH_AUTO && !dns_kernels_supported(h->H));
     h->sc.N = h->N;
     h->sc.M = h->M;
@@ -723,7 +741,8 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     dns_status st;
     if ((st = dev_alloc(h, (void **)&h->om, sizeof(cd) * nspec, "om2")) != DNS_OK) return bail(st);
     if ((st = dev_alloc(h, (void **)&h->fn, sizeof(cd) * nspec, "fnm1")) != DNS_OK) return bail(st);
-    const size_t elem = h->fp32 ? sizeof(dnsfft::cf) : sizeof(cd);
+    // bytes of one stored V / G value (FP16 pairs with engine 3)
+    const size_t elem = h->engine == 3 ? 4 : h->fp32 ? sizeof(dnsfft::cf) : sizeof(cd);
     if (h->big) {
         // BA holds Y (4 fields), X (2 planes), Qh; BB holds Z (4 fields), Yq, W.  The FP64
         // diagnostics use the same bytes: Y, X-plane 0 and rows R in BA; Z, line-FFT
@@ -743,6 +762,9 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     } else {
         if ((st = dev_alloc(h, &h->V, elem * 4 * nmid, "z-inverse fields")) != DNS_OK) return bail(st);
         if ((st = dev_alloc(h, &h->G, elem * nmid, "x-forward product")) != DNS_OK) return bail(st);
+        if (h->engine == 3 &&
+            (st = dev_alloc(h, (void **)&h->sexp, sizeof(unsigned) * (size_t)(h->H + h->M), "storage exponents")) != DNS_OK)
+            return bail(st);
     }
     if ((st = dev_alloc(h, (void **)&h->tw, sizeof(cd) * h->M, "twiddles")) != DNS_OK) return bail(st);
     const size_t ncfl = h->big ? (size_t)(h->M / 2) * dns_big_cfl_tiles(h->M) : (size_t)h->M;
@@ -803,7 +825,7 @@ extern "C" void dns_destroy(dns_handle *h)
     if (h->big) h->phys = nullptr;  // a view of buffer B
     void *ptrs[] = {h->om,  h->fn,     h->V,   h->G,    h->tw,  h->twT, h->twH, h->Vd,   h->phys,  h->cflrow,
                     h->partEZ, h->red, h->redu, h->BA,   h->BB,  nullptr,  h->twN, h->caz, h->cbz,
-                    h->fext, h->cfields, h->chp, h->cpart, h->cscale};
+                    h->fext, h->cfields, h->chp, h->cpart, h->cscale, h->sexp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -885,6 +907,7 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
         r.twH = h->twH;
         r.cflrow = nullptr;
         r.DX = r.DZ = 2.0 * PI / double(h->N);
+        set_storage_exponents(h, r);
         dns_status st = launch(h, KT_ROW, r);
         if (st != DNS_OK) return st;
         LaunchReq c{};
@@ -907,6 +930,7 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
         c.col.c2 = 0.5 * dt * (2.0 + h->sc.cnm1);
         c.col.c3 = -0.5 * dt * h->sc.cnm1;
         c.col.fext = h->closure ? h->fext : nullptr;
+        set_storage_exponents(h, c);
         st = launch(h, KT_COL, c);
         if (st != DNS_OK) return st;
         h->sc.t += dt;
@@ -1285,8 +1309,8 @@ extern "C" dns_status dns_kernel_timing(dns_handle *h, const char **names, int64
     if (!h || !n) return DNS_E_ARG;
     drain_timing(h);
     const double N2 = double(h->N) * double(h->N);
-    // algorithmic HBM bytes per launch, with c = bytes of a transform-precision complex
-    // (16 in FP64, 8 in FP32) and 16 for the FP64 state om, fn:
+    // algorithmic HBM bytes per launch, with c = bytes of a stored intermediate complex
+    // (16 in FP64, 8 in FP32, 4 as FP16 pairs) and 16 for the FP64 state om, fn:
     //   k_row     reads 4 fields M x H, writes 1: 5 * 0.75 N^2 * c           (60 N^2 | 30 N^2)
     //   k_col     reads G + om, fn, writes om, fn + V: 3.75 c N^2 + 32 N^2   (92 N^2 | 62 N^2)
     //   k_row_cfl reads u, w in FP64: 24 N^2
@@ -1297,7 +1321,7 @@ extern "C" dns_status dns_kernel_timing(dns_handle *h, const char **names, int64
     //   zfwd_b  1 field in (0.75 c), om, fn in and out (32 N^2)
     //   the half-size passes 1-3 (gradient pair only, FP64): zinv_a 8 N^2 + 1.5 c, zinv_b 3 c,
     //   xinv_a 2 fields in (1.5 c), 1 plane out (2.25 c)
-    const double c = h->fp32 ? 8.0 : 16.0;
+    const double c = h->engine == 3 ? 4.0 : h->fp32 ? 8.0 : 16.0;
     const double bytes[KT_COUNT] = {3.75 * c * N2,        (3.75 * c + 32.0) * N2, 24.0 * N2,
                                     (8.0 + 3.0 * c) * N2, 6.0 * c * N2,           7.5 * c * N2,
                                     5.625 * c * N2,       1.875 * c * N2,         1.5 * c * N2,

@@ -45,7 +45,8 @@ struct LaunchReq {
     int what;
     hipStream_t stream;
     int fp32;                  // 1: transforms and V, G in FP32 (cf); 0: FP64 (cd)
-    int engine;                // with fp32: 0 vector stages (cf), 1 BF16x3 (cm3), 2 FP16 (cmh) matrix-core stages
+    int engine;                // with fp32: 0 vector stages (cf), 1 BF16x3 (cm3), 2 FP16 (cmh) matrix-core
+                               // stages, 3 vector stages with V and G stored as FP16 pairs (cs)
     // column kernel
     dnsfft::cd *om, *fn;
     const void *G;             // cd or cf
@@ -65,11 +66,18 @@ struct LaunchReq {
     int out_mode;
     BatchArgs batch;           // column and row kernels (MODE 0, 1)
     int variant;               // launch configuration (0 default; see dns_kernels_variants)
+    // engine 3: the exponents of the FP16 storage -- vexp[kx] the four V fields' exponents of
+    // mode kx as bytes (written by the column kernel, read by the row kernel), gexp[z] that of
+    // G row z (written by the row kernel, read by the column kernel)
+    unsigned *vexp;
+    int *gexp;
 };
 
 int dns_kernels_supported(int H);
 // 1 if the matrix-core line transforms (engine 1, 2) are built for H
 int dns_kernels_mfma_supported(int H);
+// 1 if V and G stored as FP16 pairs (engine 3) are built for H
+int dns_kernels_h16_supported(int H);
 // number of launch configurations of the row / column kernels built for H (variant
 // 0 .. n-1 of LaunchReq); all give bitwise identical results
 int dns_kernels_variants(int H, int fp32);

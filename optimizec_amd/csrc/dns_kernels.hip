@@ -96,6 +96,8 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
 // runs of 32 / 64 / 128 bytes 1.21 / 1.14 / 1.16 ms FP64, 16 / 32 / 64 / 128 bytes
 // 0.76 / 0.683 / 0.645 / 0.633 ms FP32; the FP64 ensemble sizes (H <= 512) keep 128-byte
 // runs (the ensemble step with 64: 1.21 -> 1.23 ms).
+// (cs, FP16 pairs, keeps the element counts of cf: runs of 64 bytes; 128-byte runs timed
+// slower, row pass 0.284 -> 0.304 ms at N = 4096)
 template <class TC>
 __host__ __device__ constexpr int vp(int H)
 {
@@ -113,13 +115,70 @@ __host__ __device__ constexpr size_t vrow(int z, int H)
 // apart (with a row-major G, H elements apart).  Runs of 64 bytes timed best at N = 4096
 // (row-major G -> 64-byte runs: step 1.148 -> 1.118 ms FP64, column kernel 0.567 -> 0.535;
 // 0.640 -> 0.633 ms FP32; 128-byte runs: FP64 1.17 ms); the FP64 ensemble sizes keep
-// 128-byte runs.
+// 128-byte runs; cs keeps the element count of cf (32-byte runs; 64-byte: step 0.574 ->
+// 0.579 ms).
 template <class TC>
 __host__ __device__ constexpr int gq(int H)
 {
     int q = (sizeof(TC) == 16 && H < 1024 ? 128 : 64) / (int)sizeof(TC);
     while (H % q != 0) q /= 2;  // whole groups: Q divides H
     return q;
+}
+
+// FP16 storage of V and G (line type cs; FP32 transforms in registers and LDS).  A stored
+// value is an FP16 pair h with the value h * 2^e; e is one exponent per stored line: per
+// field and mode kx for V (the column kernel writes a mode's column of a field whole; the
+// four fields' exponents of mode kx packed as bytes into vexp[kx]), per row z for G (gexp[z]).
+// FP16 keeps a relative precision of 2^-11 at every magnitude from 2^-14 to 2^16, so the
+// exponent has only to keep the line below 2^16: it comes from an upper bound on the line's
+// values that the kernel already has before the transform -- the sum of |Re| + |Im| over the
+// spectrum the inverse transforms (V), the sum of |P| over the physical row the forward
+// transforms (G) -- scaled into [2^14, 2^15).  Values far below the line's bound lose
+// relative precision, but their absolute error stays below 2^-25 of that bound.  The sums
+// go through a wavefront sum and one LDS slot per wave, read after the transform (the
+// barriers are the transform's own); one LDS atomic max per lane instead (a bound from
+// the largest value) cost both passes 8-15%.
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int s16_exp(float bound)
+{
+    int E;
+    (void)frexpf(bound, &E);  // bound < 2^E
+    const int e = E - 15;
+    return e < -110 ? -110 : (e > 110 ? 110 : e);
+}
+__device__ __forceinline__ unsigned s16_pack(float x, float y, int e)
+{
+    const h2 t{(_Float16)ldexpf(x, -e), (_Float16)ldexpf(y, -e)};
+    return __builtin_bit_cast(unsigned, t);
+}
+__device__ __forceinline__ cs s16_unpack(unsigned raw, int e)
+{
+    const h2 t = __builtin_bit_cast(h2, raw);
+    return cs{ldexpf((float)t.x, e), ldexpf((float)t.y, e)};
+}
+// exponent of field slot f (0..3) in a packed vexp word
+__device__ __forceinline__ int s16_field_exp(unsigned w, int f) { return (int)(signed char)(w >> (8 * f)); }
+// Wavefront sum (whole 64-lane waves), in VALU lane moves only: DPP quad permutes and row
+// rotations sum each row of 16 lanes, then v_permlane32_swap and v_permlane16_swap add the
+// other rows (a lane swapped with itself: the two results hold its own and its partner's
+// value, in an order that does not matter for the sum).  Lanes may differ in the last bit
+// (association order); callers take one lane's value.  Through __shfl_xor (six LDS
+// permutes) the column pass's four field sums cost it 3%.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float wave_sum(float v)
+{
+    v += dpp_move<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_move<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_move<0x124>(v);  // row_ror 4
+    v += dpp_move<0x128>(v);  // row_ror 8
+    const auto h = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = __uint_as_float(h[0]) + __uint_as_float(h[1]);
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 template <class TC>
 __host__ __device__ constexpr size_t gpos(int z, int kx, int H)
@@ -182,12 +241,17 @@ template <class TC, int H, int C, int T, int MINW, int PLAN = 0>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
                                            const TC *__restrict__ G, TC *__restrict__ V,
                                            const TC *__restrict__ twT, const cd *__restrict__ tw, const ColArgs a,
-                                           BatchArgs bt, const TC *__restrict__ twH)
+                                           BatchArgs bt, const TC *__restrict__ twH,
+                                           const int *__restrict__ gexp, unsigned *__restrict__ vexp)
 {
     DNS_V_PAIRS(H);
     constexpr int N = 2 * H;
     constexpr int M = 3 * H;
     constexpr int PJ = C * H / T;
+    // cs: V and G stored as FP16 pairs (exponents gexp per G row, vexp per V column); one
+    // column per workgroup, unbatched
+    constexpr bool S16 = std::is_same<TC, cs>::value;
+    static_assert(!S16 || C == 1, "FP16 storage: one column per workgroup");
     // CN coefficients: the member's own under batching.  `a` itself is never written, so
     // a.fields[fi] (a run-time index) is read from the kernel arguments, not from a
     // private copy of the struct in scratch
@@ -270,7 +334,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
         for (int e = tid; e < 3 * C * H; e += T) {
             const int c = e % C, q = e / C;
             const int p = q / 3, z2 = q - 3 * p;
-            lds[(z2 * C + c) * LS + pidx(p)] = G[gpos<TC>(3 * zpos(p) + z2, kx0 + c, H)];
+            const int z = 3 * zpos(p) + z2;
+            if constexpr (S16)
+                lds[(z2 * C + c) * LS + pidx(p)] =
+                    s16_unpack(reinterpret_cast<const unsigned *>(G)[gpos<TC>(z, kx0 + c, H)], gexp[z]);
+            else
+                lds[(z2 * C + c) * LS + pidx(p)] = G[gpos<TC>(z, kx0 + c, H)];
         }
         // om, fn of the owned modes (lo: iz = k1, hi: iz = k1 + H), in flight during the
         // forward transforms when the twiddles come from LDS
@@ -367,6 +436,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
     // are transformed together, then each column of V (column-major, z contiguous) is
     // written whole
     const int nf = a.nfields;
+    __shared__ float redf[S16 ? T / 64 : 1];  // cs: per-wave partial bounds of a field's column
+    unsigned vw = 0;                          // cs: the column's packed field exponents
 #pragma unroll 1
     for (int fi = 0; fi < nf; ++fi) {
         const int f = a.fields[fi];
@@ -376,6 +447,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
         // kernel 0.37 -> 0.33 ms; FP64 at H = 1024 is not faster with it)
         constexpr bool FOPQ = (opq_col<TC>(H) && (!std::is_same<TC, cd>::value || H == 2048)) || (PLAN & 1) != 0;
         const int tid = FOPQ ? opaque((int)threadIdx.x) : (int)threadIdx.x;
+        double l1 = 0.0;  // cs: sum of |Re| + |Im| over this thread's spectrum values
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
             const int p = tid + j * T;
@@ -392,16 +464,39 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
             lds[(0 * C + c) * LS + pidx(k1)] = from_d<TC>(add(slo, shi));
             lds[(1 * C + c) * LS + pidx(k1)] = from_d<TC>(mulc(add(slo, mulc(shi, tw[2 * H])), wM(k1)));
             lds[(2 * C + c) * LS + pidx(k1)] = from_d<TC>(mulc(add(slo, mulc(shi, tw[H])), wM(2 * k1)));
+            if constexpr (S16) l1 += fabs(slo.x) + fabs(slo.y) + fabs(shi.x) + fabs(shi.y);
+        }
+        if constexpr (S16) {
+            const float w = wave_sum((float)l1);
+            if ((threadIdx.x & 63) == 0) redf[threadIdx.x >> 6] = w;
         }
         __syncthreads();
         lines(std::true_type{});
-        TC *out = V + (size_t)fi * a.vstride;
-        for (int e = tid; e < C * M; e += T) {
-            const int c = e / M, zz = e - c * M;
-            const int z1 = zz / 3, z2 = zz - 3 * z1;
-            out[vrow<TC>(zz, H) + vp<TC>(H) * (size_t)(kx0 + c)] = lds[(z2 * C + c) * LS + pidx(z1)];
+        if constexpr (S16) {
+            // the column's values are bounded by the sum of |Re| + |Im| of its spectrum
+            float bound = 0.0f;
+#pragma unroll
+            for (int w = 0; w < T / 64; ++w) bound += redf[w];
+            const int ex = s16_exp(bound);
+            unsigned *out = reinterpret_cast<unsigned *>(V) + (size_t)fi * a.vstride;
+            for (int e = tid; e < M; e += T) {
+                const int z1 = e / 3, z2 = e - 3 * z1;
+                const TC v = lds[z2 * LS + pidx(z1)];
+                out[vrow<TC>(e, H) + vp<TC>(H) * (size_t)kx0] = s16_pack(v.x, v.y, ex);
+            }
+            vw |= (unsigned)(ex & 0xff) << (8 * fi);
+        } else {
+            TC *out = V + (size_t)fi * a.vstride;
+            for (int e = tid; e < C * M; e += T) {
+                const int c = e / M, zz = e - c * M;
+                const int z1 = zz / 3, z2 = zz - 3 * z1;
+                out[vrow<TC>(zz, H) + vp<TC>(H) * (size_t)(kx0 + c)] = lds[(z2 * C + c) * LS + pidx(z1)];
+            }
         }
         __syncthreads();
+    }
+    if constexpr (S16) {
+        if (threadIdx.x == 0) vexp[kx0] = vw;
     }
 }
 
@@ -578,9 +673,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1))) void k_c
 template <class TC, int H, int T, int MODE, int MINW, int PLAN = 0>
 __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_t vstride, TC *__restrict__ G,
                                            const TC *__restrict__ tw, double *__restrict__ cflrow,
-                                           double DX, double DZ, BatchArgs bt, const TC *__restrict__ twH)
+                                           double DX, double DZ, BatchArgs bt, const TC *__restrict__ twH,
+                                           const unsigned *__restrict__ vexp, int *__restrict__ gexp)
 {
     DNS_V_PAIRS(H);
+    // cs: V and G stored as FP16 pairs (exponents vexp per V column, gexp per G row); the
+    // step's row pass only, unbatched
+    constexpr bool S16 = std::is_same<TC, cs>::value;
+    static_assert(!S16 || MODE == 0, "FP16 storage: the step's row pass");
     {
         const size_t b = blockIdx.y;
         V += b * bt.s_v;
@@ -595,8 +695,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
     static_assert(H % T == 0 || H < T, "pair slots must tile the half row");
     __shared__ TC lds[3 * LS];
     __shared__ TC twl[H];  // W_H^j = W_M^{3j}; W_M^e = W_H^{e / 3} W_M^{e % 3}
-    __shared__ double red[T / 64];
-    constexpr int M = 3 * H;
+    __shared__ double red[T / 64];  // MODE 1: per-wave CFL maxima; cs: per-wave |P| sums
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     // rows in XCD-aware order: neighbouring rows share the 128-byte lines of the
@@ -628,23 +727,46 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
     // raw inputs of the two field pairs: the second pair's loads are issued before the
     // first pair's transforms (whose twiddles come from LDS), so they are in flight
     // while those run.  Lane l (< 32) of a wave takes mode k = q, lane l ^ 32 mode H - q.
-    TC ra0[PJ], rb0[PJ], ra1[PJ], rb1[PJ];
-    auto issue = [&](const TC *fa, TC *a, TC *b) {
-        const TC *fb = fa + vstride;
+    // (cs: the raw FP16 pairs, unpacked where they are used, so the loads stay in flight; the
+    // packed exponents of each lane's modes)
+    using RAW = typename std::conditional<S16, unsigned, TC>::type;
+    RAW ra0[PJ], rb0[PJ], ra1[PJ], rb1[PJ];
+    unsigned vx[S16 ? PJ : 1];
+    auto issue = [&](int f0, RAW *a, RAW *b) {
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
             const int q = (tid >> 6) * 32 + (lane & 31) + j * (T / 2);
             const int k = lane < 32 ? q : (q == 0 ? H / 2 : H - q);
-            TC va = zero, vb = zero;  // a branch, not a select (a select of the two sources
-            if (q < H / 2) {          // becomes a flat load from a scratch copy of zero)
-                va = fa[vp<TC>(H) * (size_t)k];
-                vb = fb[vp<TC>(H) * (size_t)k];
+            if constexpr (S16) {
+                const unsigned *fa = reinterpret_cast<const unsigned *>(V) + f0 * vstride + vrow<TC>(z, H);
+                unsigned va = 0u, vb = 0u, w = 0u;
+                if (q < H / 2) {
+                    va = fa[vp<TC>(H) * (size_t)k];
+                    vb = fa[vstride + vp<TC>(H) * (size_t)k];
+                    if (f0 == 0) w = vexp[k];
+                }
+                if (f0 == 0) vx[j] = w;
+                a[j] = va;
+                b[j] = vb;
+            } else {
+                const TC *fa = row + f0 * vstride;
+                TC va = zero, vb = zero;  // a branch, not a select (a select of the two sources
+                if (q < H / 2) {          // becomes a flat load from a scratch copy of zero)
+                    va = fa[vp<TC>(H) * (size_t)k];
+                    vb = fa[vstride + vp<TC>(H) * (size_t)k];
+                }
+                a[j] = va;
+                b[j] = vb;
             }
-            a[j] = va;
-            b[j] = vb;
         }
     };
-    issue(row, ra0, rb0);
+    auto value = [&](RAW r, int f, int j) -> TC {
+        if constexpr (S16)
+            return s16_unpack(r, s16_field_exp(vx[j], f));
+        else
+            return r;
+    };
+    issue(0, ra0, rb0);
 #pragma unroll
     for (int ph = 0; ph < NPH; ++ph) {
 #pragma unroll
@@ -653,7 +775,8 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             const bool valid = q < H / 2;
             const bool hi = lane >= 32;
             const int k = !hi ? q : (q == 0 ? H / 2 : H - q);
-            const TC av = ph == 0 ? ra0[j] : ra1[j], bv = ph == 0 ? rb0[j] : rb1[j];
+            const TC av = value(ph == 0 ? ra0[j] : ra1[j], 2 * ph, j);
+            const TC bv = value(ph == 0 ? rb0[j] : rb1[j], 2 * ph + 1, j);
             // partner lane (l ^ 32) holds H - k, except in the q = 0 pair (0 and H/2)
             TC am{swap_halves(av.x), swap_halves(av.y)};
             TC bm{swap_halves(bv.x), swap_halves(bv.y)};
@@ -676,12 +799,13 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             }
         }
         if constexpr (NPH == 2) {
-            if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
+            if (ph == 0) issue(2, ra1, rb1);
         }
         __syncthreads();
         // vector lines in place: position p holds x1 = ip_idx(p) (the product below is pointwise
         // and the forward takes that order back); matrix-core lines in natural order
         fft_lines_to_pos<H, LS, 3, T, true, (PLAN & 2) ? 16 : 8, opq_row<TC>(H) || (PLAN & 1) != 0, ip_row<TC>(H)>(lds, twl, 1);
+        float l1 = 0.0f;  // cs: sum of |P| over this thread's points of the row
 #pragma unroll
         for (int j = 0; j < PO; ++j) {
             const int x1 = tid + j * T;
@@ -708,7 +832,14 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
                     }
                     lds[pidx(x1)] = TC{P[0], P[1]};
                     lds[LS + pidx(x1)] = TC{P[2], RT(0)};
+                    if constexpr (S16) l1 += fabsf(P[0]) + fabsf(P[1]) + fabsf(P[2]);
                 }
+            }
+        }
+        if constexpr (S16) {
+            if (ph == NPH - 1) {
+                const float w = wave_sum(l1);
+                if (lane == 0) red[tid >> 6] = w;
             }
         }
         __syncthreads();
@@ -721,6 +852,15 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
         TC *B = lds + LS;
         fft_lines_from_pos<H, LS, 2, T, false, (PLAN & 2) ? 16 : 8, opq_row<TC>(H) || (PLAN & 1) != 0, ip_row<TC>(H)>(lds, twl, 1);
         const RT half = RT(0.5);
+        int ex = 0;
+        if constexpr (S16) {
+            // the row's spectrum is bounded by the sum of |P| over the row
+            float bound = 0.0f;
+#pragma unroll
+            for (int w = 0; w < T / 64; ++w) bound += (float)red[w];
+            ex = s16_exp(bound);
+            if (tid == 0) gexp[z] = ex;
+        }
         for (int k = tid; k < H; k += T) {
             const TC za = A[pidx(k)];
             const TC zm = conj(A[pidx(k == 0 ? 0 : H - k)]);
@@ -728,7 +868,11 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
             const TC F0{half * s.x, half * s.y};
             const TC F1{half * d.y, -half * d.x};  // d / (2 i)
             const TC F2 = B[pidx(k)];
-            G[gpos<TC>(z, k, H)] = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
+            const TC g = add(F0, add(mul(F1, wM(k)), mul(F2, wM(2 * k))));
+            if constexpr (S16)
+                reinterpret_cast<unsigned *>(G)[gpos<TC>(z, k, H)] = s16_pack(g.x, g.y, ex);
+            else
+                G[gpos<TC>(z, k, H)] = g;
         }
     }
 
@@ -915,7 +1059,6 @@ __global__ __launch_bounds__(T, MINW) void k_row_mr(const TC *__restrict__ V, si
     static_assert(T % 64 == 0 && 32 % R == 0 && (H / 2) % QW == 0 && (R * H) % T == 0, "row-group tiling");
     __shared__ TC lds[3 * R * LS];
     __shared__ TC twl[H];
-    constexpr int M = 3 * H;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int z0 = xcd_tile(blockIdx.x, gridDim.x) * R;
@@ -1256,7 +1399,10 @@ DNS_CFG(2048, 1, 1024, 1024)
 template <class TC, int H, int T, int LDS_BYTES>
 constexpr int col_min_waves()
 {
-    return (std::is_same<TC, cf>::value && H == 2048 && T == 1024 && 2 * LDS_BYTES <= 160 * 1024) ? 8 : 1;
+    return ((std::is_same<TC, cf>::value || std::is_same<TC, cs>::value) && H == 2048 && T == 1024 &&
+            2 * LDS_BYTES <= 160 * 1024)
+               ? 8
+               : 1;
 }
 
 // Launch variants (the ensemble's autotuner times them): the column tile and the
@@ -1333,7 +1479,8 @@ hipError_t launch_prec(const LaunchReq &r)
             default: break;
         }
     }
-    if constexpr (H == 2048 && std::is_same<TC, cf>::value) return launch_cfg<TC, H, Cfg32_2048>(r);
+    if constexpr (H == 2048 && (std::is_same<TC, cf>::value || std::is_same<TC, cs>::value))
+        return launch_cfg<TC, H, Cfg32_2048>(r);
     return launch_cfg<TC, H, Cfg<H>>(r);
 }
 
@@ -1346,6 +1493,13 @@ hipError_t launch_cfg(const LaunchReq &r)
     constexpr int WR = 1;
     const TC *twT = static_cast<const TC *>(r.twT);
     const unsigned nb = r.batch.count > 1 ? (unsigned)r.batch.count : 1u;
+    constexpr bool S16 = std::is_same<TC, cs>::value;
+    if constexpr (S16) {
+        // FP16 storage: the step's row and column passes of one solver, four V fields at most
+        if (nb != 1 || !r.vexp || !r.gexp || (r.what != LAUNCH_ROW && r.what != LAUNCH_COL) ||
+            (r.what == LAUNCH_COL && r.col.nfields > 4))
+            return hipErrorInvalidValue;
+    }
     switch (r.what) {
         case LAUNCH_COL:
             if constexpr (H == 2048 && std::is_same<TC, cd>::value && std::is_same<Q, Cfg<H>>::value) {
@@ -1362,7 +1516,7 @@ hipError_t launch_cfg(const LaunchReq &r)
             }
             hipLaunchKernelGGL((k_col<TC, H, Q::C, Q::TC, WC, Q::plan>), dim3(H / Q::C, nb), dim3(Q::TC), 0, r.stream, r.om,
                                r.fn, static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col, r.batch,
-                               static_cast<const TC *>(r.twH));
+                               static_cast<const TC *>(r.twH), r.gexp, r.vexp);
             break;
         case LAUNCH_ROW:
             // the step's row pass computes no CFL (LAUNCH_ROW_CFL does)
@@ -1388,13 +1542,14 @@ hipError_t launch_cfg(const LaunchReq &r)
                 }
                 hipLaunchKernelGGL((k_row<TC, H, Q::TR, 0, WR, Q::plan>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
                                    static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
-                                   r.DX, r.DZ, r.batch, static_cast<const TC *>(r.twH));
+                                   r.DX, r.DZ, r.batch, static_cast<const TC *>(r.twH), r.vexp, r.gexp);
             }
             break;
         case LAUNCH_ROW_CFL:
-            hipLaunchKernelGGL((k_row<TC, H, Q::TR, 1, WR>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
-                               static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
-                               r.DX, r.DZ, r.batch, static_cast<const TC *>(r.twH));
+            if constexpr (!S16)
+                hipLaunchKernelGGL((k_row<TC, H, Q::TR, 1, WR>), dim3(M, nb), dim3(Q::TR), 0, r.stream,
+                                   static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT, r.cflrow,
+                                   r.DX, r.DZ, r.batch, static_cast<const TC *>(r.twH), nullptr, nullptr);
             break;
         case LAUNCH_ROW_PHYS:
             hipLaunchKernelGGL((k_row_phys<H, Q::TR>), dim3(M), dim3(Q::TR), 0, r.stream, r.Va, r.Vb, r.tw,
@@ -1409,6 +1564,8 @@ hipError_t launch_cfg(const LaunchReq &r)
 // the matrix-core line transforms (cm3, cmh) are built for the N=4096 grid and the sizes
 // their oracle tests use
 constexpr bool mfma_built(int H) { return H == 128 || H == 256 || H == 512 || H == 1024 || H == 2048; }
+// FP16 storage of V and G (cs) is built for the N=4096 grid
+constexpr bool h16_built(int H) { return H == 2048; }
 
 template <int H>
 hipError_t launch_all(const LaunchReq &r)
@@ -1417,6 +1574,10 @@ hipError_t launch_all(const LaunchReq &r)
         if constexpr (mfma_built(H)) {
             if (r.engine == 1) return launch_prec<cm3, H>(r);
             if (r.engine == 2) return launch_prec<cmh, H>(r);
+        }
+        if (r.engine == 3) {
+            if constexpr (h16_built(H)) return launch_prec<cs, H>(r);
+            return hipErrorInvalidValue;
         }
         if (r.engine != 0) return hipErrorInvalidValue;
         if constexpr (H >= 64) return launch_prec<cf, H>(r);
@@ -1435,6 +1596,8 @@ int dns_kernels_variants(int H, int fp32)
 int dns_kernels_variant_bitwise(int v) { return (v >= 0 && v < kNumVariants - kNumPlanVariants) ? 1 : 0; }
 
 int dns_kernels_mfma_supported(int H) { return mfma_built(H) ? 1 : 0; }
+
+int dns_kernels_h16_supported(int H) { return h16_built(H) ? 1 : 0; }
 
 int dns_kernels_supported(int H)
 {

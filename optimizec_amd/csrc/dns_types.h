@@ -24,4 +24,11 @@ struct cmh {
     float x, y;
 };
 
+// FP32 complex values of vector-stage transforms (as cf) whose intermediates between the
+// two kernels of a step, V and G, are stored as FP16 pairs under a power-of-two scale per
+// stored line (4 bytes per value; see the fused kernels)
+struct cs {
+    float x, y;
+};
+
 }  // namespace dnsfft

@@ -54,6 +54,7 @@ __device__ __forceinline__ cf to_f(cd a) { return cf{float(a.x), float(a.y)}; }
 __device__ __forceinline__ cd to_d(cf a) { return cd{double(a.x), double(a.y)}; }
 __device__ __forceinline__ cd to_d(cm3 a) { return cd{double(a.x), double(a.y)}; }
 __device__ __forceinline__ cd to_d(cmh a) { return cd{double(a.x), double(a.y)}; }
+__device__ __forceinline__ cd to_d(cs a) { return cd{double(a.x), double(a.y)}; }
 __device__ __forceinline__ cd to_d(cd a) { return a; }
 template <class C>
 __device__ __forceinline__ C from_d(cd a);
@@ -65,6 +66,8 @@ template <>
 __device__ __forceinline__ cm3 from_d<cm3>(cd a) { return cm3{float(a.x), float(a.y)}; }
 template <>
 __device__ __forceinline__ cmh from_d<cmh>(cd a) { return cmh{float(a.x), float(a.y)}; }
+template <>
+__device__ __forceinline__ cs from_d<cs>(cd a) { return cs{float(a.x), float(a.y)}; }
 
 // 0: vector radix stages; 1: three-BF16-product DFT-matrix stages (cm3); 2: FP16 (cmh)
 template <class C>
@@ -92,7 +95,7 @@ __device__ __forceinline__ int opaque(int x)
 // Value of lane l ^ 32 of the wavefront (the other half).  Doubles go through
 // v_permlane32_swap, a VALU lane swap, instead of the two LDS permutes of a 64-bit
 // __shfl_xor (FP64 row kernels at N = 4096: 0.713 -> 0.706 ms; the ensemble step -1%); the
-// swap of a register with itself leaves lanes 0-31 of the first result and lanes 32-63 of the
+// swap of a register with itself leaves lanes 32-63 of the first result and lanes 0-31 of the
 // second holding the other half's values.  Floats keep the one LDS permute (the FP32 row
 // kernel timed 2% slower with the swap).  Blocks of whole wavefronts (lane = threadIdx.x mod 64).
 __device__ __forceinline__ unsigned swap_halves_u32(unsigned x, bool upper)

@@ -11,8 +11,9 @@
 // Environment: PHASE_TIMING, BENCH_WARMUP_STEPS, RESTART_STEPS, MOV, MOV_DIR (as in
 // mem_cuda.cu:232-243, 1680-1806), plus DNS_IC=pao|k4exp (initial condition; the
 // reference's PAO-k^-3 by default), DNS_SEED (seed of the k4exp spectrum) and
-// DNS_PREC=fp64|fp32|bf16x3|fp16 (FP32 transforms, FP64 state; bf16x3 / fp16 run the
-// transforms' radix stages on the matrix cores; FP64 by default), DNS_PATH=fused|multipass
+// DNS_PREC=fp64|fp32|bf16x3|fp16|fp16store (FP32 transforms, FP64 state; bf16x3 / fp16 run
+// the transforms' radix stages on the matrix cores, fp16store keeps the intermediates as scaled
+// FP16; FP64 by default), DNS_PATH=fused|multipass
 // (kernel path where both are built; automatic by default).  MOV frames
 // go to $MOV_DIR/hip_<N>_<K0>_<Re>_<CFL>_U<UPDATE>/ (MOV_DIR defaults to ./movies; the
 // reference's default is a home directory of its author and its prefix "cuda_").
@@ -621,16 +622,20 @@ int main(int argc, char **argv)
     // DNS_PREC=fp32 (or mixed): FP32 transforms and intermediates, FP64 state, CN/AB2, dt, E/Z
     // DNS_PREC=bf16x3 / fp16: the same with the line transforms' radix stages as DFT-matrix
     // products on the matrix cores (three BF16 products, or one scaled FP16 product)
+    // DNS_PREC=fp16store: FP32 transforms with the intermediates stored as scaled FP16 pairs
     const char *prec_env = std::getenv("DNS_PREC");
     const int prec = !prec_env                                                    ? DNS_PREC_FP64
                      : (iequals(prec_env, "fp32") || iequals(prec_env, "mixed")) ? DNS_PREC_FP32
                      : iequals(prec_env, "bf16x3")                               ? DNS_PREC_BF16X3
                      : iequals(prec_env, "fp16")                                 ? DNS_PREC_FP16
+                     : iequals(prec_env, "fp16store")                            ? DNS_PREC_FP16_STORE
                                                                                  : DNS_PREC_FP64;
     if (prec == DNS_PREC_FP32) std::printf(" precision: FP32 transforms, FP64 state (DNS_PREC=%s)\n", prec_env);
     if (prec == DNS_PREC_BF16X3 || prec == DNS_PREC_FP16)
         std::printf(" precision: FP32 transforms with %s matrix-core butterflies, FP64 state (DNS_PREC=%s)\n",
                     prec == DNS_PREC_BF16X3 ? "3xBF16" : "FP16", prec_env);
+    if (prec == DNS_PREC_FP16_STORE)
+        std::printf(" precision: FP32 transforms, intermediates as scaled FP16, FP64 state (DNS_PREC=%s)\n", prec_env);
     // DNS_PATH=fused / multipass picks the kernel path where both are built (default: auto)
     const char *path_env = std::getenv("DNS_PATH");
     const int path = !path_env ? DNS_PATH_AUTO

@@ -90,6 +90,13 @@ def test_bad_arguments_are_rejected():
     assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 2  # no fused kernels for N=8192
     cfg = oa.Config(512, 1e3, 4.0, 0.1, 0, 0, 9)
     assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 1  # unknown path
+    # FP16 storage of the intermediates: N=4096 on the fused path only
+    for N, path in ((1024, oa.PATH_AUTO), (4096, oa.PATH_MULTIPASS), (29400, oa.PATH_AUTO)):
+        cfg = oa.Config(N, 1e3, 4.0, 0.1, oa.PREC_FP16_STORE, 0, path)
+        assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 2, (N, path)
+        assert b"FP16 storage" in lib.dns_last_error(None)
+    cfg = oa.Config(512, 1e3, 4.0, 0.1, oa.PREC_FP16_STORE + 1, 0, 0)
+    assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 2  # precision not built
     with pytest.raises(oa.DnsError):
         oa.ic_k4exp_host(30, 4.0, 0)
 
