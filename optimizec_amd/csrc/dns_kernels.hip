@@ -471,13 +471,17 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
             if ((threadIdx.x & 63) == 0) redf[threadIdx.x >> 6] = w;
         }
         __syncthreads();
-        lines(std::true_type{});
+        // cs: the column's values are bounded by the sum of |Re| + |Im| of its spectrum (read
+        // before the transform, which hides the reads' latency)
+        int ex = 0;
         if constexpr (S16) {
-            // the column's values are bounded by the sum of |Re| + |Im| of its spectrum
             float bound = 0.0f;
 #pragma unroll
             for (int w = 0; w < T / 64; ++w) bound += redf[w];
-            const int ex = s16_exp(bound);
+            ex = __builtin_amdgcn_readfirstlane(s16_exp(bound));  // the same in every lane: a scalar
+        }
+        lines(std::true_type{});
+        if constexpr (S16) {
             unsigned *out = reinterpret_cast<unsigned *>(V) + (size_t)fi * a.vstride;
             for (int e = tid; e < M; e += T) {
                 const int z1 = e / 3, z2 = e - 3 * z1;
@@ -850,17 +854,18 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
         // A = P0 + i P1, B = P2), then X[k] = F0 + W_M^k F1 + W_M^{2k} F2 for k < H
         TC *A = lds;
         TC *B = lds + LS;
-        fft_lines_from_pos<H, LS, 2, T, false, (PLAN & 2) ? 16 : 8, opq_row<TC>(H) || (PLAN & 1) != 0, ip_row<TC>(H)>(lds, twl, 1);
-        const RT half = RT(0.5);
+        // cs: the row's spectrum is bounded by the sum of |P| over the row (read before the
+        // transform, which hides the reads' latency)
         int ex = 0;
         if constexpr (S16) {
-            // the row's spectrum is bounded by the sum of |P| over the row
             float bound = 0.0f;
 #pragma unroll
             for (int w = 0; w < T / 64; ++w) bound += (float)red[w];
-            ex = s16_exp(bound);
+            ex = __builtin_amdgcn_readfirstlane(s16_exp(bound));  // the same in every lane: a scalar
             if (tid == 0) gexp[z] = ex;
         }
+        fft_lines_from_pos<H, LS, 2, T, false, (PLAN & 2) ? 16 : 8, opq_row<TC>(H) || (PLAN & 1) != 0, ip_row<TC>(H)>(lds, twl, 1);
+        const RT half = RT(0.5);
         for (int k = tid; k < H; k += T) {
             const TC za = A[pidx(k)];
             const TC zm = conj(A[pidx(k == 0 ? 0 : H - k)]);
