@@ -45,6 +45,11 @@ UPDATE = 100
 # TIMING_SAMPLES timed launches whatever --steps is: a pair costs about 7 us of stream time,
 # so timing every launch of the two kernels would add about 14 us to a 0.67 ms step.
 TIMING_SAMPLES = 10
+# Before the W warmup steps of an N=4096 leg the GPU steps without pause for SETTLE_S seconds
+# (untimed): the parity read before it leaves the GPU idle for tens of milliseconds, and with
+# only a few warmup steps after such a pause the timed steps ran 3-14% slower (FP64 1.150 vs
+# 1.119 ms, FP32 0.709 vs 0.646 ms at --steps 20 --warmup 5 vs 300) while the clocks came back.
+SETTLE_S = 0.3
 # algorithmic HBM bytes of one FP64 step: N^2 (2.5 c_s + 7.5 c_i) with c_s = c_i = 16 B
 # (inverse z-pass reads om and writes 4 fields, the row pass reads them and writes one,
 # the forward z-pass + CN/AB2 reads it and om, fn and writes om, fn)
@@ -104,6 +109,11 @@ def oracle_host_bytes(N: int) -> float:
     H, M, NK = N // 2, 3 * N // 2, 3 * N // 4 + 1
     nc = N * H
     return 16.0 * (3 * nc + nc + M * NK) + 8.0 * (2 * M * M + 3 * M * M)
+
+
+def note(msg: str) -> None:
+    """A progress line on stderr as each leg ends (the JSON line alone goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(N: int, steps: int, gpu: dict, big: bool = True):
@@ -226,6 +236,7 @@ def big_leg(N: int) -> dict:
     om0 = s0.om2()
     s0.close()
     ms64, ph64, gb64, E, Z, om64, dev64 = run(oa.PREC_FP64)
+    note(f"N={N} fp64: {ms64:.3f} ms/step")
     ref = float(np.linalg.norm(om64))
     change = float(np.linalg.norm(om64 - om0))
     del om0
@@ -238,6 +249,7 @@ def big_leg(N: int) -> dict:
             continue
         d = float(np.linalg.norm(om - om64))
         del om
+        note(f"N={N} {name}: {ms:.3f} ms/step")
         variants[name] = {"ms_per_step": round(ms, 3), "speedup_vs_fp64": round(ms64 / ms, 3), "phase_ms": ph,
                           "device_GB": round(dev / 1e9, 1),
                           "rel_l2_vorticity_err_vs_fp64_step41": d / ref,
@@ -405,6 +417,13 @@ def main() -> int:
     every = max(1, args.steps // TIMING_SAMPLES)
 
     def timed(s):
+        settled = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < SETTLE_S:
+            advance(s, 10)
+            s.synchronize()
+            settled += 10
+        first = s.scalars().it + args.warmup + 1
         advance(s, args.warmup)
         s.synchronize()
         s.set_kernel_timing(True, every=every)
@@ -416,13 +435,14 @@ def main() -> int:
         w = time.perf_counter() - t0
         k = s.kernel_timing()
         s.set_kernel_timing(False)
+        timed_steps[0] = (first, settled)
         return w, k
 
     # each leg runs the config to the parity checkpoint first (step 100, NEXTDT at it = 1 and
-    # 100), reads the field for the comparisons below and the oracle leg, and then times the
-    # W warmup + K steps that follow on the same solver: the timed steps are steps
-    # 101 + W .. 100 + W + K of the config run, on a GPU that has already been stepping
+    # 100), reads the field for the comparisons below and the oracle leg, steps on for
+    # SETTLE_S seconds, and then times the W warmup + K steps that follow on the same solver
     PARITY_STEPS = 100
+    timed_steps = [(0, 0)]  # (first timed step, untimed settle steps) of the last leg
 
     def leg(prec):
         s = make(prec)
@@ -437,6 +457,8 @@ def main() -> int:
     gpu_states["om0"] = a0.om2()
     a0.close()
     s, gpu_states["om64"], wall, kt = leg(oa.PREC_FP64)
+    first64, settled64 = timed_steps[0]
+    note(f"N={N} fp64: {1e3 * wall / args.steps:.4f} ms/step")
     E, Z = s.energy_enstrophy()
     finite = math.isfinite(E) and math.isfinite(Z)
     sc = s.scalars()
@@ -458,6 +480,7 @@ def main() -> int:
             sv.close()
             gpu_states["om_" + name] = om
             msv = 1e3 * wallv / args.steps
+            note(f"N={N} {name}: {msv:.4f} ms/step")
             variants[name] = {
                 "ms_per_step": round(msv, 4), "speedup_vs_fp64": round((1e3 * wall / args.steps) / msv, 3),
                 "rel_l2_vorticity_err_vs_fp64_step100": float(np.linalg.norm(om - oa64) / np.linalg.norm(oa64)),
@@ -515,8 +538,9 @@ def main() -> int:
                           "floor_ms": round(step_bytes / (HBM_PEAK_GBS * 1e9) * 1e3, 4)},
         "kernels": per_kernel,
         "kernel_timing": f"HIP events on every {every}th launch of each kernel in the timed region",
-        "timed_steps": f"{PARITY_STEPS + args.warmup + 1} .. {PARITY_STEPS + args.warmup + args.steps} of the "
-                       f"config run (steps 1 .. {PARITY_STEPS}: the parity checkpoint, then {args.warmup} warmup)",
+        "timed_steps": f"{first64} .. {first64 + args.steps - 1} of the config run (steps 1 .. {PARITY_STEPS}: "
+                       f"the parity checkpoint, then {settled64} steps in {SETTLE_S} s to steady clocks, then "
+                       f"{args.warmup} warmup)",
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": _traffic_from_profile(dom, N)},
@@ -528,11 +552,13 @@ def main() -> int:
         out["n29400" if args.big_N == 29400 else f"n{args.big_N}"] = big_leg(args.big_N)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg()
+        note("ensemble done")
     if not args.no_closure:
         from optimizec_amd import closure_eval
 
         c = closure_eval.evaluate(coarse=64, ratio=8, Re=1e5, K0=8.0, seed=1000, coarse_steps=714,
                                   ensemble="small", train_iters=1500)
+        note("closure (N=64 vs N=512) done")
         out["closure"] = {"workload": c["workload"], "ensemble": c["ensemble"], "training": c["training"],
                           "dns_seconds": c["dns_seconds"],
                           "runs": {k: {q: v[q] for q in ("seconds", "spectrum_rel_err_rms", "spectrum_rel_err_max",
@@ -544,6 +570,7 @@ def main() -> int:
                                    ensemble="config", train_iters=4000,
                                    dns_cache=os.path.join(ROOT, "tests", "golden",
                                                           "closure_dns_N8192_Re1e6_K08_seed0.npz"))
+        note("closure configuration (N=1024 vs N=8192) done")
         keys = ("seconds", "spectrum_rel_err_rms", "spectrum_rel_err_max", "spectrum_rel_err_max_resolved",
                 "spectrum_rel_err_max_tail", "enstrophy_rel_err_max", "final_energy_ratio")
         out["closure_config"] = {"workload": c5["workload"], "ensemble": c5["ensemble"], "training": c5["training"],
