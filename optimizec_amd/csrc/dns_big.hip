@@ -165,9 +165,20 @@ __device__ __forceinline__ cd spec_of(int f, cd om, int ix, int iz, int N)
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Kernels
-// launch a 1-D grid and take logical block xcd_order(b): consecutive logical blocks --
-// neighbouring tiles, which share 128-byte lines at their edges -- run on one XCD.
-__device__ __forceinline__ int xcd_order(int b, int n) { return (n % 8 == 0) ? (b % 8) * (n / 8) + b / 8 : b; }
+// Launch xcd_blocks(n) blocks for n logical blocks and take logical block xcd_order(b, n):
+// workgroups go round-robin over the 8 XCDs (each with its own L2), so XCD x = b mod 8 runs
+// the consecutive logical blocks [x per, (x + 1) per) -- neighbouring tiles, which share
+// 128-byte lines at their edges, meet in one L2.  The grid is rounded up to whole sets of 8
+// and the padding blocks (-1) return at once; without the padding a grid of n blocks with n
+// mod 8 != 0 ran unmapped, with neighbours on different XCDs, and the passes that read lines
+// shared by neighbouring tiles fetched them twice (x-forward B 1.26x, z-forward A 1.25x of
+// their algorithmic bytes).
+__host__ __device__ constexpr unsigned xcd_blocks(unsigned n) { return (n + 7) / 8 * 8; }
+__device__ __forceinline__ int xcd_order(int b, int n)
+{
+    const int per = (n + 7) / 8, lb = (b % 8) * per + b / 8;
+    return lb < n ? lb : -1;
+}
 
 // W_M^{e} from the table, conjugated for the inverse direction
 template <class TC, bool INV>
@@ -217,8 +228,12 @@ __global__ __launch_bounds__(T) void k_zinv_a(const cd *__restrict__ om, TC *__r
     __shared__ TC w1[L1], w2[L2], wb[L1];
     const int N = 2 * H;
     const int tiles = (H + BZ - 1) / BZ;
-    const int lb = xcd_order(blockIdx.x, gridDim.x);
-    const int kx0 = (lb % tiles) * BZ, n2 = (lb / tiles) % L2, pr = lb / (tiles * L2);
+    // the field pairs of one (tile, n2) are neighbouring blocks: the second reads its om
+    // values from the L2 (with the pair slowest, om was read from HBM once per pair)
+    const int lb = xcd_order(blockIdx.x, L2 * tiles * fl.pairs);
+    if (lb < 0) return;
+    const int pr = lb % fl.pairs, rest = lb / fl.pairs;
+    const int kx0 = (rest % tiles) * BZ, n2 = (rest / tiles) % L2;
     const int fa = fl.f[2 * pr], fb = fl.f[2 * pr + 1];
     // all of a thread's loads first (independent, in flight together), then the LDS writes
     constexpr int CNT = BZ * L1, IT = (CNT + T - 1) / T;
@@ -274,14 +289,15 @@ __global__ __launch_bounds__(T) void k_zinv_a(const cd *__restrict__ om, TC *__r
 // ---- pass 2: z pass B -> fields on the physical rows ------------------------------------
 template <class TC, int M_, int T>
 __global__ __launch_bounds__(T) void k_zinv_b(const TC *__restrict__ Y, TC *__restrict__ Z, const TC *__restrict__ tw,
-                                               int H, size_t hs)
+                                               int H, size_t hs, int pairs)
 {
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, BZ = bz_of<TC>(), LS = padded_len(L2);
     __shared__ TC lds[2 * BZ * LS];
     __shared__ TC w2[L2];
     const int tiles = (H + BZ - 1) / BZ;
-    const int lb = xcd_order(blockIdx.x, gridDim.x);
+    const int lb = xcd_order(blockIdx.x, L1 * tiles * pairs);
+    if (lb < 0) return;
     const int kx0 = (lb % tiles) * BZ, k1 = (lb / tiles) % L1, pr = lb / (tiles * L1);
     const TC *ya = Y + (size_t)(2 * pr) * M_ * hs, *yb = ya + (size_t)M_ * hs;
     constexpr int CNT = BZ * L2, IT = (CNT + T - 1) / T;
@@ -348,14 +364,15 @@ __device__ __forceinline__ TC row_spec(const TC *__restrict__ a, const TC *__res
 // mirror slot (conj a + i conj b); the band H <= kx <= M - H is zero.
 template <class TC, int M_, int T>
 __global__ __launch_bounds__(T) void k_xinv_a(const TC *__restrict__ Z, TC *__restrict__ X, const TC *__restrict__ tw,
-                                               int H, size_t hs, size_t ms)
+                                               int H, size_t hs, size_t ms, int planes)
 {
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, CP = bz_of<TC>(), LS = padded_len(L1);
     constexpr int NPT = (L2 / 2 - 1 + CP - 1) / CP;  // pair tiles; tile NPT holds columns 0, L2/2
     __shared__ TC lds[2 * CP * LS];
     __shared__ TC w1[L1], w2[L2], wb[L1];
-    const int lb = xcd_order(blockIdx.x, gridDim.x);
+    const int lb = xcd_order(blockIdx.x, (NPT + 1) * M_ * planes);
+    if (lb < 0) return;
     const int t = lb % (NPT + 1), z = (lb / (NPT + 1)) % M_, p = lb / ((NPT + 1) * M_);
     const TC *a = Z + ((size_t)(2 * p) * M_ + z) * hs;
     const TC *b = a + (size_t)M_ * hs;
@@ -521,7 +538,8 @@ __global__ __launch_bounds__(T) void k_xfwd_b(const TC *__restrict__ Yq, TC *__r
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, C = S::C5, LS = padded_len(L1);
     __shared__ TC lds[C * LS];
-    const int lb = xcd_order(blockIdx.x, gridDim.x);
+    const int lb = xcd_order(blockIdx.x, (M_ / 2) * (L2 / C));
+    if (lb < 0) return;
     const int m20 = (lb % (L2 / C)) * C, q = lb / (L2 / C);
     constexpr int CNT = C * L1, IT = (CNT + T - 1) / T;
     TC yv[IT];
@@ -556,7 +574,8 @@ __global__ __launch_bounds__(T) void k_zfwd_a(const TC *__restrict__ Qh, TC *__r
     __shared__ TC w1[L1], w2[L2], wb[L1];
     using RT = decltype(TC::x);
     const int tiles = (H + BZ - 1) / BZ;
-    const int lb = xcd_order(blockIdx.x, gridDim.x);
+    const int lb = xcd_order(blockIdx.x, (L2 / 2) * tiles);
+    if (lb < 0) return;
     const int kx0 = (lb % tiles) * BZ, j = lb / tiles;  // n2 = 2 j + r: rows z of parity r
     const RT half = RT(0.5);
     constexpr int CNT = BZ * L1, IT = (CNT + T - 1) / T;
@@ -612,7 +631,8 @@ __global__ __launch_bounds__(T) void k_zfwd_b(const TC *__restrict__ W, cd *__re
     __shared__ TC w2[L2];
     const int N = 2 * H;
     const int tiles = (H + BZ - 1) / BZ;
-    const int lb = xcd_order(blockIdx.x, gridDim.x);
+    // plain block order: its tiles share no lines (XCD-mapped: 7.47 -> 7.63 ms at N = 29400)
+    const int lb = blockIdx.x;
     const int kx0 = (lb % tiles) * BZ, k1 = lb / tiles;
     constexpr int CNT = BZ * L2, IT = (CNT + T - 1) / T;
     TabFetch<TC, L2, T> tf2;
@@ -789,17 +809,17 @@ hipError_t launch_m(const BigReq &r)
     hipStream_t s = r.stream;
     switch (r.what) {
         case BIG_ZINV_A:
-            hipLaunchKernelGGL((k_zinv_a<TC, M_, kTZ>), dim3(S::L2 * tiles * (unsigned)r.fields.pairs), dim3(kTZ), 0, s, r.om,
+            hipLaunchKernelGGL((k_zinv_a<TC, M_, kTZ>), dim3(xcd_blocks(S::L2 * tiles * (unsigned)r.fields.pairs)), dim3(kTZ), 0, s, r.om,
                                static_cast<TC *>(r.b), tw, r.fields, H, r.hs);
             break;
         case BIG_ZINV_B:
-            hipLaunchKernelGGL((k_zinv_b<TC, M_, kTZ>), dim3(S::L1 * tiles * (unsigned)r.fields.pairs), dim3(kTZ), 0, s,
-                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs);
+            hipLaunchKernelGGL((k_zinv_b<TC, M_, kTZ>), dim3(xcd_blocks(S::L1 * tiles * (unsigned)r.fields.pairs)), dim3(kTZ), 0, s,
+                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs, r.fields.pairs);
             break;
         case BIG_XINV_A: {
             constexpr int NPT = (S::L2 / 2 - 1 + BZ - 1) / BZ;
-            hipLaunchKernelGGL((k_xinv_a<TC, M_, txa_of<TC, M_>()>), dim3((NPT + 1) * M_ * (unsigned)r.planes), dim3(txa_of<TC, M_>()), 0, s,
-                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs, r.ms);
+            hipLaunchKernelGGL((k_xinv_a<TC, M_, txa_of<TC, M_>()>), dim3(xcd_blocks((NPT + 1) * M_ * (unsigned)r.planes)), dim3(txa_of<TC, M_>()), 0, s,
+                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs, r.ms, r.planes);
             break;
         }
         case BIG_XINV_B:
@@ -811,11 +831,11 @@ hipError_t launch_m(const BigReq &r)
                                static_cast<const TC *>(r.a), nullptr, tw, r.ms, r.cflrow, r.DX, r.DZ);
             break;
         case BIG_XFWD_B:
-            hipLaunchKernelGGL((k_xfwd_b<TC, M_, kTXF>), dim3((M_ / 2) * (S::L2 / S::C5)), dim3(kTXF), 0, s,
+            hipLaunchKernelGGL((k_xfwd_b<TC, M_, kTXF>), dim3(xcd_blocks((M_ / 2) * (S::L2 / S::C5))), dim3(kTXF), 0, s,
                                static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.ms);
             break;
         case BIG_ZFWD_A:
-            hipLaunchKernelGGL((k_zfwd_a<TC, M_, kTZ>), dim3((S::L2 / 2) * tiles), dim3(kTZ), 0, s,
+            hipLaunchKernelGGL((k_zfwd_a<TC, M_, kTZ>), dim3(xcd_blocks((S::L2 / 2) * tiles)), dim3(kTZ), 0, s,
                                static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs, r.ms);
             break;
         case BIG_ZFWD_B:
