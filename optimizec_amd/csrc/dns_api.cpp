@@ -82,6 +82,7 @@ struct dns_handle {
                                 // 3 vector with V, G stored as FP16 pairs
     unsigned *sexp = nullptr;   // engine 3: the storage exponents, vexp (H) then gexp (M)
     bool big = false;           // multi-pass path (dns_big.hip) instead of the fused kernels
+    bool div = false;           // fused FP64 kernels in the u_i u_j form: V = (u, w), G = (D^, C^)
     void *BA = nullptr;         // multi-pass work buffers (dns_big.h layouts): Y, X, Qh and the
     void *BB = nullptr;         // diagnostics rows in BA; Z, Yq, W, diagnostics plane in BB
     size_t ba_bytes = 0, bb_bytes = 0;
@@ -411,6 +412,10 @@ ColArgs inverse_args(dns_handle *h, int nf, const int *fields, size_t vstride)
     return a;
 }
 
+// the z-inverted fields the step's row kernel reads: u, w (u_i u_j form) or u, w,
+// d_x omega, d_z omega (advective form)
+int step_fields(const dns_handle *h) { return h->div ? 2 : 4; }
+
 // the FP16 storage exponents of V (per mode) and G (per row), engine 3
 void set_storage_exponents(dns_handle *h, LaunchReq &r)
 {
@@ -438,7 +443,7 @@ dns_status refresh_v(dns_handle *h)
     r.tw = h->tw;
     r.twT = h->twT;
     r.twH = h->twH;
-    r.col = inverse_args(h, 4, f4, (size_t)h->M * h->H);
+    r.col = inverse_args(h, step_fields(h), f4, (size_t)h->M * h->H);
     set_storage_exponents(h, r);
     dns_status st = launch(h, -1, r);
     if (st == DNS_OK) h->v_valid = true;
@@ -660,6 +665,7 @@ extern "C" int dns_supported_path(int N, int path)
         case DNS_PATH_AUTO: return (fused || big) ? 1 : 0;
         case DNS_PATH_FUSED: return fused ? 1 : 0;
         case DNS_PATH_MULTIPASS: return big ? 1 : 0;
+        case DNS_PATH_FUSED_UIUJ: return (fused && dns_kernels_uiuj_supported(N / 2)) ? 1 : 0;
         default: return 0;
     }
 }
@@ -676,7 +682,7 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     if (!cfg || !out) return fail(nullptr, DNS_E_ARG, "dns_create: null argument");
     *out = nullptr;
     if (cfg->N < 4 || cfg->N % 4 != 0) return fail(nullptr, DNS_E_ARG, "N=%d must be a positive multiple of 4", cfg->N);
-    if (cfg->path < DNS_PATH_AUTO || cfg->path > DNS_PATH_MULTIPASS)
+    if (cfg->path < DNS_PATH_AUTO || cfg->path > DNS_PATH_FUSED_UIUJ)
         return fail(nullptr, DNS_E_ARG, "unknown path %d", cfg->path);
     if (!dns_supported_path(cfg->N, cfg->path))
         return fail(nullptr, DNS_E_UNSUPPORTED, "N=%d: no kernels built for H=N/2=%d, M=3N/2=%d on path %d", cfg->N,
@@ -692,6 +698,9 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
             return fail(nullptr, DNS_E_UNSUPPORTED, "matrix-core line transforms are not built for N=%d on path %d",
                         cfg->N, cfg->path);
     }
+    if (cfg->path == DNS_PATH_FUSED_UIUJ && (cfg->precision != DNS_PREC_FP64 || !dns_kernels_uiuj_supported(cfg->N / 2)))
+        return fail(nullptr, DNS_E_UNSUPPORTED, "the u_i u_j form is built for FP64 at N=4096, not N=%d precision %d",
+                    cfg->N, cfg->precision);
     if (cfg->precision == DNS_PREC_FP16_STORE &&
         (cfg->path == DNS_PATH_MULTIPASS || !dns_kernels_supported(cfg->N / 2) || !dns_kernels_h16_supported(cfg->N / 2)))
         return fail(nullptr, DNS_E_UNSUPPORTED, "FP16 storage of the intermediates is built for N=4096 on the fused path, not N=%d path %d",
@@ -760,8 +769,12 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
         if ((st = dev_alloc(h, &h->BB, bb, "multi-pass buffer B")) != DNS_OK) return bail(st);
         h->phys = static_cast<double *>(h->BB);
     } else {
-        if ((st = dev_alloc(h, &h->V, elem * 4 * nmid, "z-inverse fields")) != DNS_OK) return bail(st);
-        if ((st = dev_alloc(h, &h->G, elem * nmid, "x-forward product")) != DNS_OK) return bail(st);
+        // V: the step's z-inverted fields; G: the x-forward products (D^, C^ in the u_i u_j
+        // form, the one advective product otherwise)
+        h->div = cfg->path == DNS_PATH_FUSED_UIUJ;
+        if ((st = dev_alloc(h, &h->V, elem * step_fields(h) * nmid, "z-inverse fields")) != DNS_OK) return bail(st);
+        if ((st = dev_alloc(h, &h->G, elem * (h->div ? 2 : 1) * nmid, "x-forward products")) != DNS_OK)
+            return bail(st);
         if (h->engine == 3 &&
             (st = dev_alloc(h, (void **)&h->sexp, sizeof(unsigned) * (size_t)(h->H + h->M), "storage exponents")) != DNS_OK)
             return bail(st);
@@ -907,6 +920,7 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
         r.twH = h->twH;
         r.cflrow = nullptr;
         r.DX = r.DZ = 2.0 * PI / double(h->N);
+        r.form = h->div ? 1 : 0;
         set_storage_exponents(h, r);
         dns_status st = launch(h, KT_ROW, r);
         if (st != DNS_OK) return st;
@@ -922,7 +936,7 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
         c.twT = h->twT;
         c.twH = h->twH;
         c.col.do_forward = 1;
-        c.col.nfields = 4;
+        c.col.nfields = step_fields(h);
         for (int i = 0; i < 4; ++i) c.col.fields[i] = f4[i];
         c.col.vstride = vs;
         c.col.divxz = 1.0 / (double(h->M) * double(h->M));
@@ -930,6 +944,7 @@ extern "C" dns_status dns_step(dns_handle *h, int nsteps)
         c.col.c2 = 0.5 * dt * (2.0 + h->sc.cnm1);
         c.col.c3 = -0.5 * dt * h->sc.cnm1;
         c.col.fext = h->closure ? h->fext : nullptr;
+        c.form = h->div ? 1 : 0;
         set_storage_exponents(h, c);
         st = launch(h, KT_COL, c);
         if (st != DNS_OK) return st;
@@ -1313,6 +1328,7 @@ extern "C" dns_status dns_kernel_timing(dns_handle *h, const char **names, int64
     // (16 in FP64, 8 in FP32, 4 as FP16 pairs) and 16 for the FP64 state om, fn:
     //   k_row     reads 4 fields M x H, writes 1: 5 * 0.75 N^2 * c           (60 N^2 | 30 N^2)
     //   k_col     reads G + om, fn, writes om, fn + V: 3.75 c N^2 + 32 N^2   (92 N^2 | 62 N^2)
+    //   u_i u_j form (V = u, w; G = D^, C^): k_row 4 * 0.75 N^2 * c, k_col 3 c N^2 + 32 N^2
     //   k_row_cfl reads u, w in FP64: 24 N^2
     // multi-pass passes (state 16 B; c per intermediate element; M H = 0.75 N^2, M^2 = 2.25 N^2):
     //   zinv_a  om once (8 N^2) in, 4 fields out (3 c)    zinv_b  4 fields in and out (6 c)
@@ -1322,7 +1338,8 @@ extern "C" dns_status dns_kernel_timing(dns_handle *h, const char **names, int64
     //   the half-size passes 1-3 (gradient pair only, FP64): zinv_a 8 N^2 + 1.5 c, zinv_b 3 c,
     //   xinv_a 2 fields in (1.5 c), 1 plane out (2.25 c)
     const double c = h->engine == 3 ? 4.0 : h->fp32 ? 8.0 : 16.0;
-    const double bytes[KT_COUNT] = {3.75 * c * N2,        (3.75 * c + 32.0) * N2, 24.0 * N2,
+    const double fr = h->div ? 3.0 : 3.75;  // the row and column kernels' intermediates, in c N^2
+    const double bytes[KT_COUNT] = {fr * c * N2,          (fr * c + 32.0) * N2,   24.0 * N2,
                                     (8.0 + 3.0 * c) * N2, 6.0 * c * N2,           7.5 * c * N2,
                                     5.625 * c * N2,       1.875 * c * N2,         1.5 * c * N2,
                                     (0.75 * c + 32.0) * N2, (8.0 + 1.5 * c) * N2, 3.0 * c * N2,

@@ -71,6 +71,10 @@ struct LaunchReq {
     // G row z (written by the row kernel, read by the column kernel)
     unsigned *vexp;
     int *gexp;
+    // the nonlinear term's form of the step's row and column kernels: 0 advective (V: u, w,
+    // d_x omega, d_z omega; G: one product), 1 u_i u_j (V: u, w; G: D^ = (u^2 - w^2)^ and
+    // C^ = (u w)^ at G + vstride; FP64 at H = 2048, see dns_kernels_uiuj_supported)
+    int form;
 };
 
 int dns_kernels_supported(int H);
@@ -78,6 +82,8 @@ int dns_kernels_supported(int H);
 int dns_kernels_mfma_supported(int H);
 // 1 if V and G stored as FP16 pairs (engine 3) are built for H
 int dns_kernels_h16_supported(int H);
+// 1 if the FP64 step kernels for H are built in the u_i u_j form too (LaunchReq::form = 1)
+int dns_kernels_uiuj_supported(int H);
 // number of launch configurations of the row / column kernels built for H (variant
 // 0 .. n-1 of LaunchReq); all give bitwise identical results
 int dns_kernels_variants(int H, int fp32);

@@ -185,6 +185,19 @@ __host__ __device__ constexpr size_t gpos(int z, int kx, int H)
 {
     return (size_t)(kx / gq<TC>(H)) * (3 * (size_t)H * gq<TC>(H)) + (size_t)z * gq<TC>(H) + (kx % gq<TC>(H));
 }
+// The nonlinear term's form.  Advective (u d_x omega + w d_z omega, the default): four
+// z-inverted fields (u, w, d_x omega, d_z omega) in V, one real product in G.  The u_i u_j
+// (divergence) form, the reference's own (step2b_cuda.cu:137-173, step3_cuda.cu:75-81):
+// FN = [alpha gamma (u^2 - w^2)^ + (gamma^2 - alpha^2) (u w)^] / M^2 from two fields in V
+// (u, w) and two real products in G (D = u^2 - w^2, C = u w; planes 0, 1): fewer bytes and
+// line transforms per step (row: 3 inverse + 3 forward lines instead of 6 + 2; column: 6
+// forward + 6 inverse instead of 3 + 12; FP64 intermediates 96 instead of 120 N^2 bytes).  Its
+// wavenumber factors multiply the products' rounding by up to K^2 after the transforms (the
+// advective form applies them before), which FP64 absorbs (8.8e-14 from the advective oracle
+// after 100 steps at N = 4096) and FP32 transforms do not (4.8e-5, over the 1e-5 bar): built
+// for FP64 at H = 2048 (N = 4096) only.
+__host__ __device__ constexpr bool uiuj_built(int H) { return H == 2048; }
+
 // the M = 3H rows group exactly
 #define DNS_V_PAIRS(H_) static_assert((3 * (H_)) % vp<cd>(H_) == 0 && (3 * (H_)) % vp<dnsfft::cf>(H_) == 0, "V row groups")
 
@@ -237,7 +250,7 @@ constexpr bool ip_col(int H)
 }
 // PLAN (the ensemble autotuner's launch variants): bit 0 forces the opaque FFT thread index,
 // bit 1 the radix-16 plan of the line transforms (different rounding: not bitwise equal)
-template <class TC, int H, int C, int T, int MINW, int PLAN = 0>
+template <class TC, int H, int C, int T, int MINW, int PLAN = 0, bool DIV = false>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void k_col(cd *__restrict__ om, cd *__restrict__ fn,
                                            const TC *__restrict__ G, TC *__restrict__ V,
                                            const TC *__restrict__ twT, const cd *__restrict__ tw, const ColArgs a,
@@ -330,48 +343,72 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
     // registers: the (lo, hi) pair of each owned (column c, k1): iz = k1 and iz = k1 + H
     cd xlo[PJ], xhi[PJ];
 
+    // u_i u_j form (DIV): G holds D^ (plane 0) and C^ (plane 1); FN = (alpha gamma D^ +
+    // (gamma^2 - alpha^2) C^) / M^2, accumulated in xlo / xhi plane by plane
+    static_assert(!DIV || (uiuj_built(H) && std::is_same<TC, cd>::value), "u_i u_j form: FP64 at H = 2048");
+    constexpr int NPL = DIV ? 2 : 1;
     if (a.do_forward) {
-        for (int e = tid; e < 3 * C * H; e += T) {
-            const int c = e % C, q = e / C;
-            const int p = q / 3, z2 = q - 3 * p;
-            const int z = 3 * zpos(p) + z2;
-            if constexpr (S16)
-                lds[(z2 * C + c) * LS + pidx(p)] =
-                    s16_unpack(reinterpret_cast<const unsigned *>(G)[gpos<TC>(z, kx0 + c, H)], gexp[z]);
-            else
-                lds[(z2 * C + c) * LS + pidx(p)] = G[gpos<TC>(z, kx0 + c, H)];
-        }
         // om, fn of the owned modes (lo: iz = k1, hi: iz = k1 + H), in flight during the
         // forward transforms when the twiddles come from LDS
         cd olo[PJ], ohi[PJ], flo[PJ], fhi[PJ];
-        if constexpr (PF) {
+        // u_i u_j form: plane 0's share of FN, held across plane 1's transform
+        cd plo[DIV ? PJ : 1], phi[DIV ? PJ : 1];
+#pragma unroll
+        for (int pl = 0; pl < NPL; ++pl) {
+            if (pl > 0) __syncthreads();  // the previous plane's combine has read the lines
+            for (int e = tid; e < 3 * C * H; e += T) {
+                const int c = e % C, q = e / C;
+                const int p = q / 3, z2 = q - 3 * p;
+                const int z = 3 * zpos(p) + z2;
+                const size_t go = (size_t)pl * a.vstride + gpos<TC>(z, kx0 + c, H);
+                if constexpr (S16)
+                    lds[(z2 * C + c) * LS + pidx(p)] = s16_unpack(reinterpret_cast<const unsigned *>(G)[go], gexp[z]);
+                else
+                    lds[(z2 * C + c) * LS + pidx(p)] = G[go];
+            }
+            if constexpr (PF && !DIV) {
+#pragma unroll
+                for (int j = 0; j < PJ; ++j) {
+                    const int p = tid + j * T;
+                    const int c = p / H, k1 = p - (p / H) * H;
+                    const size_t io = (size_t)(kx0 + c) * N + k1;
+                    olo[j] = om[io];
+                    ohi[j] = om[io + H];
+                    flo[j] = fn[io];
+                    fhi[j] = fn[io + H];
+                }
+            }
+            __syncthreads();
+            lines_fwd();
+            // radix-3 combine in FP64 (the FFT outputs are widened)
+            const cd w3a = tw[2 * H];  // W_3^2
+            const cd w3b = tw[H];      // W_3^4 = W_3
 #pragma unroll
             for (int j = 0; j < PJ; ++j) {
                 const int p = tid + j * T;
                 const int c = p / H, k1 = p - (p / H) * H;
-                const size_t io = (size_t)(kx0 + c) * N + k1;
-                olo[j] = om[io];
-                ohi[j] = om[io + H];
-                flo[j] = fn[io];
-                fhi[j] = fn[io + H];
+                const cd F0 = to_d(lds[(0 * C + c) * LS + pidx(k1)]);
+                const cd F1 = mul(to_d(lds[(1 * C + c) * LS + pidx(k1)]), wM(k1));
+                const cd F2 = mul(to_d(lds[(2 * C + c) * LS + pidx(k1)]), wM(2 * k1));
+                const cd Xlo = add(F0, add(F1, F2));
+                const cd Xhi = add(F0, add(mul(F1, w3a), mul(F2, w3b)));
+                if constexpr (DIV) {
+                    const double ax = double(kx0 + c), glo = gamma_of(k1, N), ghi = gamma_of(k1 + H, N);
+                    if (pl == 0) {
+                        plo[j] = scl(Xlo, glo * ax * a.divxz);
+                        phi[j] = scl(Xhi, ghi * ax * a.divxz);
+                    } else {
+                        xlo[j] = add(plo[j], scl(Xlo, (glo * glo - ax * ax) * a.divxz));
+                        xhi[j] = add(phi[j], scl(Xhi, (ghi * ghi - ax * ax) * a.divxz));
+                    }
+                } else {
+                    xlo[j] = Xlo;
+                    xhi[j] = Xhi;
+                }
             }
         }
-        __syncthreads();
-        lines_fwd();
-        // radix-3 combine in FP64 (the FFT outputs are widened)
-        const cd w3a = tw[2 * H];  // W_3^2
-        const cd w3b = tw[H];      // W_3^4 = W_3
-#pragma unroll
-        for (int j = 0; j < PJ; ++j) {
-            const int p = tid + j * T;
-            const int c = p / H, k1 = p - (p / H) * H;
-            const cd F0 = to_d(lds[(0 * C + c) * LS + pidx(k1)]);
-            const cd F1 = mul(to_d(lds[(1 * C + c) * LS + pidx(k1)]), wM(k1));
-            const cd F2 = mul(to_d(lds[(2 * C + c) * LS + pidx(k1)]), wM(2 * k1));
-            xlo[j] = add(F0, add(F1, F2));
-            xhi[j] = add(F0, add(mul(F1, w3a), mul(F2, w3b)));
-        }
-        // CN/AB2 update (step3_cuda.cu:75-100): FN = -P^/M^2, row kz = -N/2 forced to 0
+        // CN/AB2 update (step3_cuda.cu:75-100): FN = -P^/M^2 (advective; the u_i u_j form has
+        // FN in xlo / xhi already), row kz = -N/2 forced to 0
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
             const int p = tid + j * T;
@@ -384,12 +421,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MINW))) void 
                 const double gz = gamma_of(iz, N);
                 const double K2 = ax * ax + gz * gz;
                 const cd X = half ? xhi[j] : xlo[j];
-                cd FN = mk(-X.x * a.divxz, -X.y * a.divxz);
+                cd FN = DIV ? X : mk(-X.x * a.divxz, -X.y * a.divxz);
                 const size_t io = (size_t)kx * N + iz;
                 if (a.fext) FN = add(FN, a.fext[io]);
                 if (iz == H) FN = mk(0.0, 0.0);
                 cd o, f;
-                if constexpr (PF) {
+                if constexpr (PF && !DIV) {
                     o = half ? ohi[j] : olo[j];
                     f = half ? fhi[j] : flo[j];
                 } else {
@@ -908,7 +945,7 @@ __global__ __launch_bounds__(T, MINW) void k_row(const TC *__restrict__ V, size_
 // + b / 8, so at any time the 32 workgroups of an XCD work on consecutive rows that share
 // the 128-byte lines of the column-major V in that XCD's L2.
 // =====================================================================================
-template <class TC, int H, int T, int RPW>
+template <class TC, int H, int T, int RPW, bool DIV = false>
 __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t vstride, TC *__restrict__ G,
                                                 const TC *__restrict__ tw, const TC *__restrict__ twH)
 {
@@ -956,6 +993,9 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
             b[j] = vb;
         }
     };
+    // u_i u_j form (DIV): one field pair (u, w), products D = u^2 - w^2, C = u w
+    static_assert(!DIV || uiuj_built(H), "u_i u_j form: H = 2048");
+    constexpr int NPH = DIV ? 1 : 2;
     issue(V + vrow<TC>(row_of(0), H), ra0, rb0);
 #pragma unroll 1
     for (int i = 0; i < RPW; ++i) {
@@ -966,7 +1006,7 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
         const int z = row_of(i);
         const TC *row = V + vrow<TC>(z, H);
 #pragma unroll
-        for (int ph = 0; ph < 2; ++ph) {
+        for (int ph = 0; ph < NPH; ++ph) {
 #pragma unroll
             for (int j = 0; j < PJ; ++j) {
                 const int q = (tid >> 6) * 32 + (lane & 31) + j * (T / 2);
@@ -994,7 +1034,7 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
                     lds[2 * LS + pidx(k)] = mulc(add(zlo, mulc(zhi, w3_2)), wM(2 * k));
                 }
             }
-            if (ph == 0) issue(row + 2 * vstride, ra1, rb1);
+            if (!DIV && ph == 0) issue(row + 2 * vstride, ra1, rb1);
             __syncthreads();
             // in place (one barrier per stage): position p of every line holds x1 = ip_idx(p);
             // the product below is pointwise and the forward DIT takes that order back
@@ -1002,7 +1042,13 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
 #pragma unroll
             for (int j = 0; j < PO; ++j) {
                 const int x1 = tid + j * T;
-                if (ph == 0) {
+                if constexpr (DIV) {
+#pragma unroll
+                    for (int n2 = 0; n2 < 3; ++n2) {
+                        const TC v = lds[n2 * LS + pidx(x1)];
+                        lds[n2 * LS + pidx(x1)] = TC{v.x * v.x - v.y * v.y, v.x * v.y};
+                    }
+                } else if (ph == 0) {
 #pragma unroll
                     for (int n2 = 0; n2 < 3; ++n2) vel[n2][j] = lds[n2 * LS + pidx(x1)];
                 } else {
@@ -1018,13 +1064,34 @@ __global__ __launch_bounds__(T, 1) void k_row_p(const TC *__restrict__ V, size_t
             }
             // the next row's first pair, into the registers this row's pairs have freed: in
             // flight during the forward transform
-            if (ph == 1 && i + 1 < RPW) issue(V + vrow<TC>(row_of(i + 1), H), ra0, rb0);
+            if (ph == NPH - 1 && i + 1 < RPW) issue(V + vrow<TC>(row_of(i + 1), H), ra0, rb0);
             __syncthreads();
+        }
+        const RT half = RT(0.5);
+        if constexpr (DIV) {
+            // x-forward of D + i C, split into D^ and C^ (as in k_row)
+            fft_lines_from_pos<H, LS, 3, T, false, 8, opq_row<TC>(H)>(lds, twl, 1);
+            for (int k = tid; k < H; k += T) {
+                const TC X =
+                    add(lds[pidx(k)], add(mul(lds[LS + pidx(k)], wM(k)), mul(lds[2 * LS + pidx(k)], wM(2 * k))));
+                TC Xm = X;
+                if (k != 0) {
+                    const int km = H - k;
+                    Xm = add(lds[pidx(km)], add(mul(mul(lds[LS + pidx(km)], wM(km)), w3_1),
+                                                mul(mul(lds[2 * LS + pidx(km)], wM(2 * km)), w3_2)));
+                }
+                const TC zm = conj(Xm);
+                const TC sm = add(X, zm), df = sub(X, zm);
+                const size_t o = gpos<TC>(z, k, H);
+                G[o] = TC{half * sm.x, half * sm.y};
+                G[vstride + o] = TC{half * df.y, -half * df.x};
+            }
+            __syncthreads();  // the next row writes the lines this loop reads
+            continue;
         }
         TC *A = lds;
         TC *B = lds + LS;
         fft_lines_from_pos<H, LS, 2, T, false, 8, opq_row<TC>(H)>(lds, twl, 1);
-        const RT half = RT(0.5);
         for (int k = tid; k < H; k += T) {
             const TC za = A[pidx(k)];
             const TC zm = conj(A[pidx(k == 0 ? 0 : H - k)]);
@@ -1499,6 +1566,11 @@ hipError_t launch_cfg(const LaunchReq &r)
     const TC *twT = static_cast<const TC *>(r.twT);
     const unsigned nb = r.batch.count > 1 ? (unsigned)r.batch.count : 1u;
     constexpr bool S16 = std::is_same<TC, cs>::value;
+    // the u_i u_j form: FP64 step kernels at H = 2048, unbatched
+    constexpr bool UI = std::is_same<TC, cd>::value && uiuj_built(H) && std::is_same<Q, Cfg<H>>::value;
+    if (r.form != 0) {
+        if (!UI || r.form != 1 || nb != 1 || (r.what != LAUNCH_ROW && r.what != LAUNCH_COL)) return hipErrorInvalidValue;
+    }
     if constexpr (S16) {
         // FP16 storage: the step's row and column passes of one solver, four V fields at most
         if (nb != 1 || !r.vexp || !r.gexp || (r.what != LAUNCH_ROW && r.what != LAUNCH_COL) ||
@@ -1513,9 +1585,17 @@ hipError_t launch_cfg(const LaunchReq &r)
                 constexpr int CPW = 8, NW = H / CPW;
                 static_assert(NW % 8 == 0 && NW * CPW == H && Q::C == 1, "whole columns per workgroup");
                 if (nb == 1 && !r.batch.coef && r.col.do_forward && r.col.nfields >= 1) {
-                    hipLaunchKernelGGL((k_col_p<H, Q::TC, CPW>), dim3(NW), dim3(Q::TC), 0, r.stream, r.om, r.fn,
-                                       static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col,
-                                       static_cast<const TC *>(r.twH));
+                    // the u_i u_j form: the one-column kernel (the persistent one, with plane 0's
+                    // share of FN held across plane 1's transform beside its prefetch, spilled and
+                    // ran 0.486 against 0.480 ms)
+                    if (r.form == 1)
+                        hipLaunchKernelGGL((k_col<TC, H, Q::C, Q::TC, WC, Q::plan, true>), dim3(H / Q::C), dim3(Q::TC), 0,
+                                           r.stream, r.om, r.fn, static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT,
+                                           r.tw, r.col, r.batch, static_cast<const TC *>(r.twH), r.gexp, r.vexp);
+                    else
+                        hipLaunchKernelGGL((k_col_p<H, Q::TC, CPW>), dim3(NW), dim3(Q::TC), 0, r.stream, r.om, r.fn,
+                                           static_cast<const TC *>(r.G), static_cast<TC *>(r.V), twT, r.tw, r.col,
+                                           static_cast<const TC *>(r.twH));
                     break;
                 }
             }
@@ -1539,9 +1619,14 @@ hipError_t launch_cfg(const LaunchReq &r)
                     constexpr int RPW = 24, NW = M / RPW;
                     static_assert(NW % 8 == 0 && NW * RPW == M, "whole rows per workgroup, whole XCD sets");
                     if (nb == 1) {
-                        hipLaunchKernelGGL((k_row_p<TC, H, Q::TR, RPW>), dim3(NW), dim3(Q::TR), 0, r.stream,
-                                           static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT,
-                                           static_cast<const TC *>(r.twH));
+                        if (r.form == 1)
+                            hipLaunchKernelGGL((k_row_p<TC, H, Q::TR, RPW, true>), dim3(NW), dim3(Q::TR), 0, r.stream,
+                                               static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT,
+                                               static_cast<const TC *>(r.twH));
+                        else
+                            hipLaunchKernelGGL((k_row_p<TC, H, Q::TR, RPW>), dim3(NW), dim3(Q::TR), 0, r.stream,
+                                               static_cast<const TC *>(r.V), r.vstride, static_cast<TC *>(r.Gw), twT,
+                                               static_cast<const TC *>(r.twH));
                         break;
                     }
                 }
@@ -1603,6 +1688,8 @@ int dns_kernels_variant_bitwise(int v) { return (v >= 0 && v < kNumVariants - kN
 int dns_kernels_mfma_supported(int H) { return mfma_built(H) ? 1 : 0; }
 
 int dns_kernels_h16_supported(int H) { return h16_built(H) ? 1 : 0; }
+
+int dns_kernels_uiuj_supported(int H) { return uiuj_built(H) ? 1 : 0; }
 
 int dns_kernels_supported(int H)
 {

@@ -40,3 +40,15 @@ def oracle_4096(oracle_mod):
     print(f"oracle N={N}: {steps} steps", flush=True)
     ro = o.run(steps, update, form=1)
     return {"om0": om0, "om": o.om2(), "rows": ro, "steps": steps, "update": update}
+
+
+@pytest.fixture(scope="session")
+def oracle_4096_uiuj(oracle_mod):
+    """The same N=4096 run on the oracle in the reference's u_i u_j form (form 0)."""
+    N, steps, update = 4096, 100, 100
+    o = oracle_mod.Oracle(N, 1e5, 4.0, 0.1)
+    o.init_config(0)
+    o.start(form=0)
+    print(f"oracle N={N} (u_i u_j form): {steps} steps", flush=True)
+    ro = o.run(steps, update, form=0)
+    return {"om": o.om2(), "rows": ro, "steps": steps, "update": update}

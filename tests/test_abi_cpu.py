@@ -97,6 +97,14 @@ def test_bad_arguments_are_rejected():
         assert b"FP16 storage" in lib.dns_last_error(None)
     cfg = oa.Config(512, 1e3, 4.0, 0.1, oa.PREC_FP16_STORE + 1, 0, 0)
     assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 2  # precision not built
+    # the u_i u_j form of the fused kernels: FP64 at N=4096 only
+    for N, prec in ((2048, oa.PREC_FP64), (29400, oa.PREC_FP64), (4096, oa.PREC_FP32), (4096, oa.PREC_FP16_STORE)):
+        cfg = oa.Config(N, 1e3, 4.0, 0.1, prec, 0, oa.PATH_FUSED_UIUJ)
+        assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 2, (N, prec)
+    assert lib.dns_supported_path(4096, oa.PATH_FUSED_UIUJ) == 1
+    assert lib.dns_supported_path(2048, oa.PATH_FUSED_UIUJ) == 0
+    cfg = oa.Config(4096, 1e3, 4.0, 0.1, 0, 0, oa.PATH_FUSED_UIUJ + 1)
+    assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 1  # unknown path
     with pytest.raises(oa.DnsError):
         oa.ic_k4exp_host(30, 4.0, 0)
 

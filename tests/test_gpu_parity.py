@@ -159,6 +159,28 @@ def test_4096_config_100_steps_against_oracle(oracle_4096):
     s.close()
 
 
+def test_4096_uiuj_form_against_oracle(oracle_4096_uiuj, oracle_4096):
+    """The fused FP64 kernels in the reference's u_i u_j form (PATH_FUSED_UIUJ) on the N=4096
+    configuration: 100 steps against the oracle in the same form -- rel. L2 vorticity <= 1e-11,
+    t/dt/cn/CFLM/E/Z <= 1e-9 at every diagnostic step -- and against the advective oracle run
+    (the two forms agree to rounding amplified by the wavenumber factors: <= 1e-11)."""
+    s = oa.Solver(4096, 1e5, 4.0, 0.1, path=oa.PATH_FUSED_UIUJ)
+    s.init_ic(oa.IC_K4EXP, 0)
+    s.start()
+    rs = s.run(oracle_4096_uiuj["steps"], oracle_4096_uiuj["update"])
+    errs = _series_errors(rs, oracle_4096_uiuj["rows"])
+    om = s.om2()
+    err = relerr(om, oracle_4096_uiuj["om"])
+    err_adv = relerr(om, oracle_4096["om"])
+    print(f"N=4096 FP64 u_i u_j form 100 steps: rel L2 vorticity {err:.3e} (advective oracle: {err_adv:.3e}); "
+          f"series {errs}")
+    for name, e in errs.items():
+        assert e < 1e-9, (name, e)
+    assert err < 1e-11
+    assert err_adv < 1e-11
+    s.close()
+
+
 def test_4096_fp32_transforms_against_oracle(oracle_4096):
     """Mixed precision at N=4096 (FP32 transforms, FP64 state, CN/AB2, CFL/dt, E/Z) against
     the FP64 oracle after 100 steps: rel. L2 vorticity <= 1e-5.  The error is also reported
