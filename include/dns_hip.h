@@ -66,11 +66,12 @@ typedef enum {
     DNS_PATH_AUTO = 0,       /* fused kernels when built for H = N/2, else the multi-pass path   */
     DNS_PATH_FUSED = 1,      /* LDS-resident row and column kernels (N <= 4096)                  */
     DNS_PATH_MULTIPASS = 2,  /* batched four-step line FFTs through HBM (large N, e.g. 29400)    */
-    DNS_PATH_FUSED_UIUJ = 3  /* the fused kernels with the nonlinear term in the reference's u_i u_j
+    DNS_PATH_FUSED_UIUJ = 3, /* the fused kernels with the nonlinear term in the reference's u_i u_j
                                 (divergence) form, FN from (u^2 - w^2)^ and (u w)^
                                 (step2b_cuda.cu:137-173, step3_cuda.cu:75-81) instead of the
                                 advective u . grad(omega): half the z-inverted fields, two products
                                 instead of one; FP64 at N = 4096                                    */
+    DNS_PATH_MULTIPASS_UIUJ = 4 /* the multi-pass path in the u_i u_j form (FP64)                  */
 } dns_path;
 
 typedef enum {

@@ -20,12 +20,12 @@ __all__ = ["Solver", "Scalars", "DnsError", "library_path", "load_library", "ic_
            "ic_k4exp_host", "supported_n", "FIELD_U", "FIELD_W", "FIELD_OMEGA", "FIELD_PSI",
            "FIELD_KINETIC", "IC_PAO_KM3", "IC_K4EXP", "PREC_FP64", "PREC_FP32", "PREC_BF16X3", "PREC_FP16",
            "PREC_FP16_STORE",
-           "PATH_AUTO", "PATH_FUSED", "PATH_MULTIPASS", "PATH_FUSED_UIUJ"]
+           "PATH_AUTO", "PATH_FUSED", "PATH_MULTIPASS", "PATH_FUSED_UIUJ", "PATH_MULTIPASS_UIUJ"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 IC_PAO_KM3, IC_K4EXP = 0, 1
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP16, PREC_FP16_STORE = 0, 1, 2, 3, 4
-PATH_AUTO, PATH_FUSED, PATH_MULTIPASS, PATH_FUSED_UIUJ = 0, 1, 2, 3
+PATH_AUTO, PATH_FUSED, PATH_MULTIPASS, PATH_FUSED_UIUJ, PATH_MULTIPASS_UIUJ = 0, 1, 2, 3, 4
 FIELD_U, FIELD_W, FIELD_OMEGA, FIELD_PSI, FIELD_KINETIC = 0, 1, 2, 3, 4
 _STATUS = {0: "DNS_OK", 1: "DNS_E_ARG", 2: "DNS_E_UNSUPPORTED", 3: "DNS_E_HIP", 4: "DNS_E_NOMEM",
            5: "DNS_E_STATE", 6: "DNS_E_IO", 7: "DNS_E_FORMAT"}
@@ -203,8 +203,9 @@ class Solver:
         PREC_FP16_STORE (PREC_FP32 with the intermediates between the step's two kernels
         stored as power-of-two-scaled FP16 pairs; N=4096, fused path).
         path: PATH_AUTO (fused kernels where built, else multi-pass), PATH_FUSED or
-        PATH_MULTIPASS (batched four-step line FFTs through HBM), or PATH_FUSED_UIUJ (the fused
-        kernels with the nonlinear term in the reference's u_i u_j form; FP64, N=4096)."""
+        PATH_MULTIPASS (batched four-step line FFTs through HBM), or PATH_FUSED_UIUJ /
+        PATH_MULTIPASS_UIUJ (those with the nonlinear term in the reference's u_i u_j form; FP64;
+        the fused one at N=4096)."""
         L = load_library()
         cfg = Config(N, float(Re), float(K0), float(cflnum), int(precision), int(device), int(path))
         h = _P()

@@ -83,6 +83,7 @@ struct dns_handle {
     unsigned *sexp = nullptr;   // engine 3: the storage exponents, vexp (H) then gexp (M)
     bool big = false;           // multi-pass path (dns_big.hip) instead of the fused kernels
     bool div = false;           // fused FP64 kernels in the u_i u_j form: V = (u, w), G = (D^, C^)
+    bool big_uiuj = false;      // multi-pass FP64 step in the u_i u_j form (big_step_uiuj)
     void *BA = nullptr;         // multi-pass work buffers (dns_big.h layouts): Y, X, Qh and the
     void *BB = nullptr;         // diagnostics rows in BA; Z, Yq, W, diagnostics plane in BB
     size_t ba_bytes = 0, bb_bytes = 0;
@@ -252,9 +253,70 @@ dns_status big_zinv(dns_handle *h, bool fp64, const int *fields, int pairs, bool
     return big_launch(h, timed ? KT_BIG_ZINV_B : -1, r);
 }
 
+// the multi-pass step in the u_i u_j form (FP64): passes 1-3 for u, w only (skipped when the
+// NEXTDT on this om left their plane in buffer A), pass 4 forms D = u^2 - w^2 and C = u w, passes
+// 5 and 6 carry both, pass 7 forms FN from both
+dns_status big_step_uiuj(dns_handle *h, double dt)
+{
+    static const int fv[2] = {DNS_F_U, DNS_F_W};
+    const bool reuse = h->x0_valid;
+    h->x0_valid = false;
+    dns_status st;
+    BigReq r{};
+    r.tw = h->tw;
+    r.uiuj = 1;
+    if (!reuse) {
+        // passes 1-3 for one field pair: the half-size launches of the advective step's reuse
+        BigReq z{};
+        z.what = BIG_ZINV_A;
+        z.tw = h->tw;
+        z.om = h->om;
+        z.b = h->BA;
+        z.fields.pairs = 1;
+        z.fields.f[0] = fv[0];
+        z.fields.f[1] = fv[1];
+        if ((st = big_launch(h, KT_BIG_ZINV_A_HALF, z)) != DNS_OK) return st;
+        z.what = BIG_ZINV_B;
+        z.a = h->BA;
+        z.b = h->BB;
+        if ((st = big_launch(h, KT_BIG_ZINV_B_HALF, z)) != DNS_OK) return st;
+        r.what = BIG_XINV_A;
+        r.planes = 1;
+        r.a = h->BB;
+        r.b = h->BA;
+        if ((st = big_launch(h, KT_BIG_XINV_A_HALF, r)) != DNS_OK) return st;
+    }
+    r.what = BIG_XINV_B;
+    r.a = h->BA;
+    r.b = h->BB;
+    if ((st = big_launch(h, KT_BIG_XINV_B, r)) != DNS_OK) return st;
+    r.what = BIG_XFWD_B;
+    r.a = h->BB;
+    r.b = h->BA;
+    if ((st = big_launch(h, KT_BIG_XFWD_B, r)) != DNS_OK) return st;
+    const size_t e = sizeof(cd);
+    for (int p = 0; p < 2; ++p) {  // pass 6 per plane: Qh plane p (M / 2 rows) -> W field p
+        BigReq z = r;
+        z.what = BIG_ZFWD_A;
+        z.a = static_cast<const char *>(h->BA) + (size_t)p * (h->M / 2) * h->ms * e;
+        z.b = static_cast<char *>(h->BB) + (size_t)p * h->M * h->hs * e;
+        if ((st = big_launch(h, KT_BIG_ZFWD_A, z)) != DNS_OK) return st;
+    }
+    r.what = BIG_ZFWD_B;
+    r.a = h->BB;
+    r.om = h->om;
+    r.fn = h->fn;
+    r.cn.divxz = 1.0 / (double(h->M) * double(h->M));
+    r.cn.VT = 0.5 * h->sc.visc * dt;
+    r.cn.c2 = 0.5 * dt * (2.0 + h->sc.cnm1);
+    r.cn.c3 = -0.5 * dt * h->sc.cnm1;
+    return big_launch(h, KT_BIG_ZFWD_B, r);
+}
+
 // one time step of the multi-pass path (seven passes, dns_big.hip)
 dns_status big_step(dns_handle *h, double dt)
 {
+    if (h->big_uiuj) return big_step_uiuj(h, dt);
     static const int f4[4] = {DNS_F_U, DNS_F_W, DNS_F_OX, DNS_F_OZ};
     static const int fg[2] = {DNS_F_OX, DNS_F_OZ};
     const bool reuse = h->x0_valid && !h->fp32;
@@ -666,6 +728,7 @@ extern "C" int dns_supported_path(int N, int path)
         case DNS_PATH_FUSED: return fused ? 1 : 0;
         case DNS_PATH_MULTIPASS: return big ? 1 : 0;
         case DNS_PATH_FUSED_UIUJ: return (fused && dns_kernels_uiuj_supported(N / 2)) ? 1 : 0;
+        case DNS_PATH_MULTIPASS_UIUJ: return big ? 1 : 0;
         default: return 0;
     }
 }
@@ -682,7 +745,7 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
     if (!cfg || !out) return fail(nullptr, DNS_E_ARG, "dns_create: null argument");
     *out = nullptr;
     if (cfg->N < 4 || cfg->N % 4 != 0) return fail(nullptr, DNS_E_ARG, "N=%d must be a positive multiple of 4", cfg->N);
-    if (cfg->path < DNS_PATH_AUTO || cfg->path > DNS_PATH_FUSED_UIUJ)
+    if (cfg->path < DNS_PATH_AUTO || cfg->path > DNS_PATH_MULTIPASS_UIUJ)
         return fail(nullptr, DNS_E_ARG, "unknown path %d", cfg->path);
     if (!dns_supported_path(cfg->N, cfg->path))
         return fail(nullptr, DNS_E_UNSUPPORTED, "N=%d: no kernels built for H=N/2=%d, M=3N/2=%d on path %d", cfg->N,
@@ -698,6 +761,9 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
             return fail(nullptr, DNS_E_UNSUPPORTED, "matrix-core line transforms are not built for N=%d on path %d",
                         cfg->N, cfg->path);
     }
+    if (cfg->path == DNS_PATH_MULTIPASS_UIUJ && cfg->precision != DNS_PREC_FP64)
+        return fail(nullptr, DNS_E_UNSUPPORTED, "the u_i u_j form of the multi-pass step is built for FP64, not precision %d",
+                    cfg->precision);
     if (cfg->path == DNS_PATH_FUSED_UIUJ && (cfg->precision != DNS_PREC_FP64 || !dns_kernels_uiuj_supported(cfg->N / 2)))
         return fail(nullptr, DNS_E_UNSUPPORTED, "the u_i u_j form is built for FP64 at N=4096, not N=%d precision %d",
                     cfg->N, cfg->precision);
@@ -723,7 +789,9 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
                 : cfg->precision == DNS_PREC_FP16       ? 2
                 : cfg->precision == DNS_PREC_FP16_STORE ? 3
                                                         : 0;
-    h->big = cfg->path == DNS_PATH_MULTIPASS || (cfg->path == DNS_PATH_AUTO && !dns_kernels_supported(h->H));
+    h->big = cfg->path == DNS_PATH_MULTIPASS || cfg->path == DNS_PATH_MULTIPASS_UIUJ ||
+             (cfg->path == DNS_PATH_AUTO && !dns_kernels_supported(h->H));
+    h->big_uiuj = cfg->path == DNS_PATH_MULTIPASS_UIUJ;
     h->sc.N = h->N;
     h->sc.M = h->M;
     h->sc.H = h->H;
@@ -1339,10 +1407,13 @@ extern "C" dns_status dns_kernel_timing(dns_handle *h, const char **names, int64
     //   xinv_a 2 fields in (1.5 c), 1 plane out (2.25 c)
     const double c = h->engine == 3 ? 4.0 : h->fp32 ? 8.0 : 16.0;
     const double fr = h->div ? 3.0 : 3.75;  // the row and column kernels' intermediates, in c N^2
+    // the multi-pass u_i u_j form: pass 4 reads one plane and writes two half planes (4.5 c),
+    // pass 5 carries two half planes (3.75 c), pass 7 reads two fields (1.5 c + 32)
+    const bool ui = h->big_uiuj;
     const double bytes[KT_COUNT] = {fr * c * N2,          (fr * c + 32.0) * N2,   24.0 * N2,
                                     (8.0 + 3.0 * c) * N2, 6.0 * c * N2,           7.5 * c * N2,
-                                    5.625 * c * N2,       1.875 * c * N2,         1.5 * c * N2,
-                                    (0.75 * c + 32.0) * N2, (8.0 + 1.5 * c) * N2, 3.0 * c * N2,
+                                    (ui ? 4.5 : 5.625) * c * N2, (ui ? 3.75 : 1.875) * c * N2, 1.5 * c * N2,
+                                    ((ui ? 1.5 : 0.75) * c + 32.0) * N2, (8.0 + 1.5 * c) * N2, 3.0 * c * N2,
                                     3.75 * c * N2};
     const int cnt = std::min(*n, (int)KT_COUNT);
     for (int i = 0; i < cnt; ++i) {

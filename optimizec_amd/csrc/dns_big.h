@@ -68,6 +68,10 @@ struct BigReq {
     size_t lines = 0;
     size_t hs = 0, ms = 0;
     int planes = 2;
+    // FP64 step passes 4, 5, 7 in the u_i u_j form: pass 4 reads X plane 0 (u + i w) and writes
+    // Yq planes D = u^2 - w^2, C = u w (M / 2 rows each); pass 5 takes both (M rows); pass 6
+    // runs once per plane; pass 7 reads W fields D, C (M hs apart) and forms FN from both
+    int uiuj = 0;
     BigFields fields{};
     BigCnArgs cn{};
     double *cflrow = nullptr;

@@ -462,7 +462,9 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
 {
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, C = S::C4, LS = padded_len(L2);
-    constexpr int G = MODE == 0 ? 4 : 2;  // lines per k1: group g = 2 p + r (plane p, row 2q + r)
+    // lines per k1: group g = 2 p + r (plane p, row 2q + r); MODE 2 (u_i u_j form) reads plane 0
+    // (u + i w) only and forward-transforms two products per row pair
+    constexpr int G = MODE == 0 ? 4 : 2;
     __shared__ TC lds[G * C * LS];
     __shared__ TC w2[L2], wb[L1];
     __shared__ double red[T / 64];
@@ -513,6 +515,26 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
         }
         return;
     }
+    if constexpr (MODE == 2) {
+        // D = u^2 - w^2 and C = u w of rows 2q (real parts) and 2q + 1 (imaginary parts):
+        // lines c hold D, lines C + c hold C; both x-forwarded, written to Yq planes 0 and 1
+        for (int e = threadIdx.x; e < C * L2; e += T) {
+            const int c = e / L2, k2 = e % L2;
+            const TC a0 = lds[c * LS + pidx(k2)], a1 = lds[(C + c) * LS + pidx(k2)];
+            lds[c * LS + pidx(k2)] = TC{RT(a0.x * a0.x - a0.y * a0.y), RT(a1.x * a1.x - a1.y * a1.y)};
+            lds[(C + c) * LS + pidx(k2)] = TC{RT(a0.x * a0.y), RT(a1.x * a1.y)};
+        }
+        __syncthreads();
+        fft_lines<L2, LS, 2 * C, T, false, kXBR, big_opq<TC, M_>(4)>(lds, w2, 1);
+        for (int e = threadIdx.x; e < 2 * C * L2; e += T) {
+            const int pc = e / L2, m2 = e % L2;
+            const int pl = pc / C, c = pc - pl * C;
+            const int k1 = k10 + c;
+            Yq[((size_t)pl * (M_ / 2) + q) * ms + (size_t)k1 * L2 + m2] =
+                mul(lds[pc * LS + pidx(m2)], twid2<TC, L1, false>(w2, wb, k1 * m2));
+        }
+        return;
+    }
     // P = u w_x + w w_z on rows 2q (real part) and 2q + 1 (imaginary part): group 0/1 hold
     // u + i w of the two rows, group 2/3 w_x + i w_z
     for (int e = threadIdx.x; e < C * L2; e += T) {
@@ -533,12 +555,12 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
 // ---- pass 5: L1-point FFTs over k1 -> packed row-pair spectra (kept modes only) ----------
 template <class TC, int M_, int T>
 __global__ __launch_bounds__(T) void k_xfwd_b(const TC *__restrict__ Yq, TC *__restrict__ Qh, const TC *__restrict__ tw,
-                                               int H, size_t ms)
+                                               int H, size_t ms, int rows)
 {
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, C = S::C5, LS = padded_len(L1);
     __shared__ TC lds[C * LS];
-    const int lb = xcd_order(blockIdx.x, (M_ / 2) * (L2 / C));
+    const int lb = xcd_order(blockIdx.x, rows * (L2 / C));  // rows: M / 2, or M for two planes
     if (lb < 0) return;
     const int m20 = (lb % (L2 / C)) * C, q = lb / (L2 / C);
     constexpr int CNT = C * L1, IT = (CNT + T - 1) / T;
@@ -621,47 +643,58 @@ __global__ __launch_bounds__(T) void k_zfwd_a(const TC *__restrict__ Qh, TC *__r
 }
 
 // ---- pass 7: z pass B (forward), FN, CN/AB2 ----------------------------------------------
-template <class TC, int M_, int T>
+template <class TC, int M_, int T, bool UI = false>
 __global__ __launch_bounds__(T) void k_zfwd_b(const TC *__restrict__ W, cd *__restrict__ om, cd *__restrict__ fn,
                                                const TC *__restrict__ tw, int H, size_t hs, BigCnArgs a)
 {
+    // UI: the u_i u_j form -- W holds the z-pass-A outputs of D = u^2 - w^2 (field 0) and C = u w
+    // (field 1, M hs further); FN = (alpha gamma D^ + (gamma^2 - alpha^2) C^) / M^2
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, BZ = bz_of<TC>(), LS = padded_len(L2);
-    __shared__ TC lds[BZ * LS];
+    constexpr int NF = UI ? 2 : 1;
+    __shared__ TC lds[NF * BZ * LS];
     __shared__ TC w2[L2];
     const int N = 2 * H;
     const int tiles = (H + BZ - 1) / BZ;
     // plain block order: its tiles share no lines (XCD-mapped: 7.47 -> 7.63 ms at N = 29400)
     const int lb = blockIdx.x;
     const int kx0 = (lb % tiles) * BZ, k1 = lb / tiles;
-    constexpr int CNT = BZ * L2, IT = (CNT + T - 1) / T;
+    constexpr int CNT = NF * BZ * L2, IT = (CNT + T - 1) / T;
     TabFetch<TC, L2, T> tf2;
     tf2.fetch(tw, L1);
     TC wv[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
         const int e = threadIdx.x + i * T;
-        const int c = e % BZ, n2 = e / BZ;
+        const int c = e % BZ, rest = e / BZ, n2 = rest % L2, f = rest / L2;
         const int kx = kx0 + c;
-        wv[i] = (e < CNT && kx < H) ? W[ztile<TC>(k1 * L2 + n2, kx, M_)] : TC{0, 0};
+        wv[i] = (e < CNT && kx < H) ? W[(size_t)f * M_ * hs + ztile<TC>(k1 * L2 + n2, kx, M_)] : TC{0, 0};
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
         const int e = threadIdx.x + i * T;
-        if (e < CNT) lds[(e % BZ) * LS + pidx(e / BZ)] = wv[i];
+        if (e < CNT) {
+            const int c = e % BZ, rest = e / BZ, n2 = rest % L2, f = rest / L2;
+            lds[(f * BZ + c) * LS + pidx(n2)] = wv[i];
+        }
     }
     tf2.put(w2);
     __syncthreads();
-    fft_lines<L2, LS, BZ, T, false, 8, big_opq<TC, M_>(7)>(lds, w2, 1);
+    fft_lines<L2, LS, NF * BZ, T, false, 8, big_opq<TC, M_>(7)>(lds, w2, 1);
     for (int e = threadIdx.x; e < BZ * L2; e += T) {
         const int c = e % BZ, k2 = e / BZ;
         const int kx = kx0 + c, kz = k1 + L1 * k2;
         const int iz = kz < H ? kz : (kz >= 2 * H ? kz - H : -1);
         if (kx >= H || iz < 0) continue;
+        const double ax = double(kx), gz = gam(iz, N);
         const cd X = to_d(lds[c * LS + pidx(k2)]);
         cd FN{-X.x * a.divxz, -X.y * a.divxz};
+        if constexpr (UI) {
+            const cd Xc = to_d(lds[(BZ + c) * LS + pidx(k2)]);
+            const double ga = gz * ax * a.divxz, gm = (gz * gz - ax * ax) * a.divxz;
+            FN = cd{ga * X.x + gm * Xc.x, ga * X.y + gm * Xc.y};
+        }
         if (iz == H) FN = cd{0.0, 0.0};
-        const double ax = double(kx), gz = gam(iz, N);
         const double ARG = a.VT * (ax * ax + gz * gz);
         const double DEN = 1.0 + ARG, c1 = 1.0 - ARG;
         const size_t io = dns_state_pos(iz, kx, N);
@@ -823,24 +856,36 @@ hipError_t launch_m(const BigReq &r)
             break;
         }
         case BIG_XINV_B:
-            hipLaunchKernelGGL((k_xinv_b<TC, M_, 0, kTXB>), dim3(M_ / 2, S::L1 / S::C4), dim3(kTXB), 0, s,
-                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, r.ms, nullptr, 0.0, 0.0);
+            if (r.uiuj)
+                hipLaunchKernelGGL((k_xinv_b<TC, M_, 2, kTXB>), dim3(M_ / 2, S::L1 / S::C4), dim3(kTXB), 0, s,
+                                   static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, r.ms, nullptr, 0.0, 0.0);
+            else
+                hipLaunchKernelGGL((k_xinv_b<TC, M_, 0, kTXB>), dim3(M_ / 2, S::L1 / S::C4), dim3(kTXB), 0, s,
+                                   static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, r.ms, nullptr, 0.0, 0.0);
             break;
         case BIG_XINV_CFL:
             hipLaunchKernelGGL((k_xinv_b<TC, M_, 1, kTXB>), dim3(M_ / 2, S::L1 / S::C4), dim3(kTXB), 0, s,
                                static_cast<const TC *>(r.a), nullptr, tw, r.ms, r.cflrow, r.DX, r.DZ);
             break;
         case BIG_XFWD_B:
-            hipLaunchKernelGGL((k_xfwd_b<TC, M_, kTXF>), dim3(xcd_blocks((M_ / 2) * (S::L2 / S::C5))), dim3(kTXF), 0, s,
-                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.ms);
+        {
+            // the u_i u_j form's two planes (D, C) are M / 2 rows each, back to back
+            const int rows = r.uiuj ? M_ : M_ / 2;
+            hipLaunchKernelGGL((k_xfwd_b<TC, M_, kTXF>), dim3(xcd_blocks((unsigned)rows * (S::L2 / S::C5))), dim3(kTXF), 0, s,
+                               static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.ms, rows);
+        }
             break;
         case BIG_ZFWD_A:
             hipLaunchKernelGGL((k_zfwd_a<TC, M_, kTZ>), dim3(xcd_blocks((S::L2 / 2) * tiles)), dim3(kTZ), 0, s,
                                static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, H, r.hs, r.ms);
             break;
         case BIG_ZFWD_B:
-            hipLaunchKernelGGL((k_zfwd_b<TC, M_, kTZ7>), dim3(S::L1 * tiles), dim3(kTZ7), 0, s, static_cast<const TC *>(r.a),
-                               r.om, r.fn, tw, H, r.hs, r.cn);
+            if (r.uiuj)
+                hipLaunchKernelGGL((k_zfwd_b<TC, M_, kTZ7, true>), dim3(S::L1 * tiles), dim3(kTZ7), 0, s,
+                                   static_cast<const TC *>(r.a), r.om, r.fn, tw, H, r.hs, r.cn);
+            else
+                hipLaunchKernelGGL((k_zfwd_b<TC, M_, kTZ7>), dim3(S::L1 * tiles), dim3(kTZ7), 0, s, static_cast<const TC *>(r.a),
+                                   r.om, r.fn, tw, H, r.hs, r.cn);
             break;
         default:
             return hipErrorInvalidValue;
@@ -913,6 +958,7 @@ int dns_big_cfl_tiles(int M)
 hipError_t dns_big_launch(const BigReq &r)
 {
     const int M = 3 * r.H;
+    if (r.uiuj && r.fp32) return hipErrorInvalidValue;  // the u_i u_j form is built for FP64
     switch (r.what) {
         case BIG_LINE_INV: return launch_lines(r, M);
         case BIG_ROWS:

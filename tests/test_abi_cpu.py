@@ -103,7 +103,10 @@ def test_bad_arguments_are_rejected():
         assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 2, (N, prec)
     assert lib.dns_supported_path(4096, oa.PATH_FUSED_UIUJ) == 1
     assert lib.dns_supported_path(2048, oa.PATH_FUSED_UIUJ) == 0
-    cfg = oa.Config(4096, 1e3, 4.0, 0.1, 0, 0, oa.PATH_FUSED_UIUJ + 1)
+    cfg = oa.Config(29400, 1e3, 4.0, 0.1, oa.PREC_FP32, 0, oa.PATH_MULTIPASS_UIUJ)
+    assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 2  # the multi-pass u_i u_j form is FP64
+    assert lib.dns_supported_path(29400, oa.PATH_MULTIPASS_UIUJ) == 1
+    cfg = oa.Config(4096, 1e3, 4.0, 0.1, 0, 0, oa.PATH_MULTIPASS_UIUJ + 1)
     assert lib.dns_create(ctypes.byref(cfg), ctypes.byref(h)) == 1  # unknown path
     with pytest.raises(oa.DnsError):
         oa.ic_k4exp_host(30, 4.0, 0)

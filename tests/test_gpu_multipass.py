@@ -269,3 +269,66 @@ def test_29400_config_one_step_against_oracle(oracle_mod):
     err = _rel_l2_chunked(om_gpu, om_o)
     print(f"N={N} config, 1 step: rel L2 vorticity vs oracle {err:.3e}", flush=True)
     assert err < 1e-13
+
+
+MPU = oa.PATH_MULTIPASS_UIUJ
+
+
+def test_multipass_uiuj_512_100_steps(oracle_mod):
+    """The multi-pass step in the reference's u_i u_j form (PATH_MULTIPASS_UIUJ) against the
+    oracle in the same form: 100 steps at N=512 with NEXTDT every 20 (the step after each NEXTDT
+    reuses its velocity plane and skips passes 1-3)."""
+    N, steps, update = 512, 100, 20
+    s = oa.Solver(N, 1e4, 4.0, 0.1, path=MPU)
+    s.init_ic(oa.IC_K4EXP, 3)
+    o = oracle_mod.Oracle(N, 1e4, 4.0, 0.1)
+    o.init_config(3)
+    print("multipass u_i u_j N=512 100 steps", flush=True)
+    s.start()
+    o.start(form=0)
+    rs = s.run(steps, update)
+    ro = o.run(steps, update, form=0)
+    for col, name in ((1, "t"), (2, "dt"), (3, "cn"), (4, "cflm"), (5, "E"), (6, "Z")):
+        err = np.max(np.abs(rs[:, col] - ro[:, col]) / np.abs(ro[:, col]))
+        assert err < 1e-9, (name, err)
+    err = relerr(s.om2(), o.om2())
+    print(f"multipass u_i u_j N=512 100 steps: rel L2 vorticity error {err:.3e}")
+    assert err < 1e-11
+    s.close()
+    o.close()
+
+
+def test_multipass_uiuj_matches_fused_uiuj_4096():
+    N = 4096
+    print("multipass vs fused, u_i u_j form, N=4096", flush=True)
+    a = oa.Solver(N, 1e5, 4.0, 0.1, path=oa.PATH_FUSED_UIUJ)
+    b = oa.Solver(N, 1e5, 4.0, 0.1, path=MPU)
+    for s in (a, b):
+        s.init_ic(oa.IC_K4EXP, 0)
+        s.start()
+    ra, rb = a.run(5, 5), b.run(5, 5)
+    assert np.max(np.abs(ra[:, 1:] - rb[:, 1:]) / np.abs(rb[:, 1:])) < 1e-12
+    err = relerr(b.om2(), a.om2())
+    print(f"multipass vs fused u_i u_j N=4096, 5 steps: {err:.3e}")
+    assert err < 1e-13
+    a.close()
+    b.close()
+
+
+def test_multipass_uiuj_29400_against_advective():
+    """N=29400 (Re=1e7, K0=4, CFL=0.1, seed 0): 3 FP64 steps in the u_i u_j form against the
+    advective multi-pass step (the two forms agree to rounding amplified by the wavenumber
+    factors)."""
+    N = 29400
+    oms = []
+    for path in (MP, MPU):
+        print(f"N={N} path {path}: start + 3 steps", flush=True)
+        s = oa.Solver(N, 1e7, 4.0, 0.1, path=path)
+        s.init_ic(oa.IC_K4EXP, 0)
+        s.start()
+        s.step(3)
+        oms.append(s.om2())
+        s.close()
+    err = _rel_l2_chunked(oms[1], oms[0])
+    print(f"N={N}, 3 steps: u_i u_j vs advective rel L2 vorticity {err:.3e}", flush=True)
+    assert err < 1e-11
