@@ -13,8 +13,8 @@
 // reference's PAO-k^-3 by default), DNS_SEED (seed of the k4exp spectrum) and
 // DNS_PREC=fp64|fp32|bf16x3|fp16|fp16store (FP32 transforms, FP64 state; bf16x3 / fp16 run
 // the transforms' radix stages on the matrix cores, fp16store keeps the intermediates as scaled
-// FP16; FP64 by default), DNS_PATH=fused|multipass|uiuj (uiuj: the fused FP64 kernels with
-// the nonlinear term in the reference's u_i u_j form, N=4096)
+// FP16; FP64 by default), DNS_PATH=fused|multipass|uiuj|multipass-uiuj (uiuj: the FP64
+// kernels with the nonlinear term in the reference's u_i u_j form; the fused ones at N=4096)
 // (kernel path where both are built; automatic by default).  MOV frames
 // go to $MOV_DIR/hip_<N>_<K0>_<Re>_<CFL>_U<UPDATE>/ (MOV_DIR defaults to ./movies; the
 // reference's default is a home directory of its author and its prefix "cuda_").
@@ -643,11 +643,13 @@ int main(int argc, char **argv)
                      : iequals(path_env, "fused") ? DNS_PATH_FUSED
                      : iequals(path_env, "multipass") ? DNS_PATH_MULTIPASS
                      : iequals(path_env, "uiuj") ? DNS_PATH_FUSED_UIUJ
+                     : iequals(path_env, "multipass-uiuj") ? DNS_PATH_MULTIPASS_UIUJ
                                                       : DNS_PATH_AUTO;
     if (path != DNS_PATH_AUTO)
-        std::printf(" path: %s (DNS_PATH)\n", path == DNS_PATH_FUSED       ? "fused"
-                                              : path == DNS_PATH_FUSED_UIUJ ? "fused, u_i u_j form"
-                                                                            : "multi-pass");
+        std::printf(" path: %s (DNS_PATH)\n", path == DNS_PATH_FUSED           ? "fused"
+                                              : path == DNS_PATH_FUSED_UIUJ     ? "fused, u_i u_j form"
+                                              : path == DNS_PATH_MULTIPASS_UIUJ ? "multi-pass, u_i u_j form"
+                                                                                : "multi-pass");
     dns_config cfg{N, Re, K0, CFL, prec, 0, path};
     dns_status st = dns_create(&cfg, &r.h);
     if (st != DNS_OK) {
