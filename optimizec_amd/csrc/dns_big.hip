@@ -455,6 +455,10 @@ __global__ __launch_bounds__(T) void k_xinv_a(const TC *__restrict__ Z, TC *__re
 }
 
 // ---- pass 4: x pass B of a row pair, product, first forward step -------------------------
+// row pairs per pass-4 workgroup: two in the u_i u_j form (the tables and barriers of a
+// workgroup then serve twice the lines), one otherwise
+template <int M_, int MODE>
+constexpr int xb_pairs() { return MODE == 2 && M_ % 4 == 0 ? 2 : 1; }
 // MODE 0: product -> Yq;  MODE 1: CFL maxima of u, w (plane 0 only) -> cflpart
 template <class TC, int M_, int MODE, int T>
 __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__restrict__ Yq, const TC *__restrict__ tw,
@@ -463,8 +467,9 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, C = S::C4, LS = padded_len(L2);
     // lines per k1: group g = 2 p + r (plane p, row 2q + r); MODE 2 (u_i u_j form) reads plane 0
-    // (u + i w) only and forward-transforms two products per row pair
-    constexpr int G = MODE == 0 ? 4 : 2;
+    // (u + i w) of RP row pairs (rows 2 RP q + g) and forward-transforms two products per pair
+    constexpr int RP = xb_pairs<M_, MODE>();
+    constexpr int G = MODE == 0 ? 4 : 2 * RP;
     __shared__ TC lds[G * C * LS];
     __shared__ TC w2[L2], wb[L1];
     __shared__ double red[T / 64];
@@ -480,8 +485,8 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
     for (int i = 0; i < IT; ++i) {
         const int e = threadIdx.x + i * T;
         const int n2 = e % L2, rest = e / L2, c = rest % C, g = rest / C;
-        const int p = g >> 1, r = g & 1;
-        xv[i] = e < CNT ? X[((size_t)p * M_ + 2 * q + r) * ms + (size_t)(k10 + c) * L2 + n2] : TC{0, 0};
+        const int p = MODE == 2 ? 0 : g >> 1, row = MODE == 2 ? 2 * RP * q + g : 2 * q + (g & 1);
+        xv[i] = e < CNT ? X[((size_t)p * M_ + row) * ms + (size_t)(k10 + c) * L2 + n2] : TC{0, 0};
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
@@ -516,22 +521,24 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
         return;
     }
     if constexpr (MODE == 2) {
-        // D = u^2 - w^2 and C = u w of rows 2q (real parts) and 2q + 1 (imaginary parts):
-        // lines c hold D, lines C + c hold C; both x-forwarded, written to Yq planes 0 and 1
-        for (int e = threadIdx.x; e < C * L2; e += T) {
-            const int c = e / L2, k2 = e % L2;
-            const TC a0 = lds[c * LS + pidx(k2)], a1 = lds[(C + c) * LS + pidx(k2)];
-            lds[c * LS + pidx(k2)] = TC{RT(a0.x * a0.x - a0.y * a0.y), RT(a1.x * a1.x - a1.y * a1.y)};
-            lds[(C + c) * LS + pidx(k2)] = TC{RT(a0.x * a0.y), RT(a1.x * a1.y)};
+        // D = u^2 - w^2 and C = u w of rows 2j (real parts) and 2j + 1 (imaginary parts) of
+        // pair j = RP q + pp: lines 2 pp C + c hold D, lines (2 pp + 1) C + c hold C; all
+        // x-forwarded, written to Yq planes 0 (D) and 1 (C)
+        for (int e = threadIdx.x; e < RP * C * L2; e += T) {
+            const int pc = e / L2, k2 = e % L2, pp = pc / C, c = pc - pp * C;
+            const int l0 = 2 * pp * C + c, l1 = l0 + C;
+            const TC a0 = lds[l0 * LS + pidx(k2)], a1 = lds[l1 * LS + pidx(k2)];
+            lds[l0 * LS + pidx(k2)] = TC{RT(a0.x * a0.x - a0.y * a0.y), RT(a1.x * a1.x - a1.y * a1.y)};
+            lds[l1 * LS + pidx(k2)] = TC{RT(a0.x * a0.y), RT(a1.x * a1.y)};
         }
         __syncthreads();
-        fft_lines<L2, LS, 2 * C, T, false, kXBR, big_opq<TC, M_>(4)>(lds, w2, 1);
-        for (int e = threadIdx.x; e < 2 * C * L2; e += T) {
-            const int pc = e / L2, m2 = e % L2;
-            const int pl = pc / C, c = pc - pl * C;
+        fft_lines<L2, LS, 2 * RP * C, T, false, kXBR, big_opq<TC, M_>(4)>(lds, w2, 1);
+        for (int e = threadIdx.x; e < 2 * RP * C * L2; e += T) {
+            const int gc = e / L2, m2 = e % L2;
+            const int g = gc / C, c = gc - g * C, pp = g >> 1, pl = g & 1;
             const int k1 = k10 + c;
-            Yq[((size_t)pl * (M_ / 2) + q) * ms + (size_t)k1 * L2 + m2] =
-                mul(lds[pc * LS + pidx(m2)], twid2<TC, L1, false>(w2, wb, k1 * m2));
+            Yq[((size_t)pl * (M_ / 2) + RP * q + pp) * ms + (size_t)k1 * L2 + m2] =
+                mul(lds[gc * LS + pidx(m2)], twid2<TC, L1, false>(w2, wb, k1 * m2));
         }
         return;
     }
@@ -857,7 +864,7 @@ hipError_t launch_m(const BigReq &r)
         }
         case BIG_XINV_B:
             if (r.uiuj)
-                hipLaunchKernelGGL((k_xinv_b<TC, M_, 2, kTXB>), dim3(M_ / 2, S::L1 / S::C4), dim3(kTXB), 0, s,
+                hipLaunchKernelGGL((k_xinv_b<TC, M_, 2, kTXB>), dim3(M_ / (2 * xb_pairs<M_, 2>()), S::L1 / S::C4), dim3(kTXB), 0, s,
                                    static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, r.ms, nullptr, 0.0, 0.0);
             else
                 hipLaunchKernelGGL((k_xinv_b<TC, M_, 0, kTXB>), dim3(M_ / 2, S::L1 / S::C4), dim3(kTXB), 0, s,
