@@ -47,6 +47,9 @@ namespace {
 // against 64 and 256 bytes: both slower in every z pass)
 constexpr int kT = 256;
 constexpr int kTZ = 512, kTXA = 512, kTXB = 256, kTXF = 512, kTZ7 = 256;
+// pass 7 of the u_i u_j form (two fields, 64 KB of LDS per workgroup: two workgroups per CU)
+// at 512 threads: 9.74 -> 9.55 ms FP64 at N = 29400 (1024 threads: 10.8 ms)
+constexpr int kTZ7U = 512;
 // pass 3 with FP32 intermediates at N = 29400 runs 1024 threads (13.8 -> 12.6 ms; at
 // N = 8192 and 16384 it is slower, and the other FP32 passes at 256 or 1024 threads were
 // slower or equal)
@@ -888,7 +891,7 @@ hipError_t launch_m(const BigReq &r)
             break;
         case BIG_ZFWD_B:
             if (r.uiuj)
-                hipLaunchKernelGGL((k_zfwd_b<TC, M_, kTZ7, true>), dim3(S::L1 * tiles), dim3(kTZ7), 0, s,
+                hipLaunchKernelGGL((k_zfwd_b<TC, M_, kTZ7U, true>), dim3(S::L1 * tiles), dim3(kTZ7U), 0, s,
                                    static_cast<const TC *>(r.a), r.om, r.fn, tw, H, r.hs, r.cn);
             else
                 hipLaunchKernelGGL((k_zfwd_b<TC, M_, kTZ7>), dim3(S::L1 * tiles), dim3(kTZ7), 0, s, static_cast<const TC *>(r.a),
