@@ -43,10 +43,12 @@ namespace {
 // threads per workgroup: the diagnostics' line FFTs (kT) and the step's passes (kTZ: the
 // z-direction passes 1, 2, 6; kTXA: pass 3; kTXB: pass 4; kTXF: pass 5; kTZ7: pass 7),
 // chosen by timing at N = 29400 (pass 7 at 256 threads instead of 512: 7.9 -> 7.4 ms FP64,
-// 5.9 -> 5.8 ms with FP32 intermediates, same box; z tiles of 128 bytes, bz_of, timed
-// against 64 and 256 bytes: both slower in every z pass)
+// 5.9 -> 5.8 ms with FP32 intermediates, same box; pass 5 at 256 instead of 512: 5.45 ->
+// 4.55 ms FP64, 2.88 -> 2.55 ms FP32, 10.8 -> 9.15 ms in the u_i u_j form, while passes 1-3
+// and 6 at 256 were slower; z tiles of 128 bytes, bz_of, timed against 64 and 256 bytes:
+// both slower in every z pass)
 constexpr int kT = 256;
-constexpr int kTZ = 512, kTXA = 512, kTXB = 256, kTXF = 512, kTZ7 = 256;
+constexpr int kTZ = 512, kTXA = 512, kTXB = 256, kTXF = 256, kTZ7 = 256;
 // pass 7 of the u_i u_j form (two fields, 64 KB of LDS per workgroup: two workgroups per CU)
 // at 512 threads: 9.74 -> 9.55 ms FP64 at N = 29400 (1024 threads: 10.8 ms)
 constexpr int kTZ7U = 512;
