@@ -452,7 +452,7 @@ dns_status big_cflm(dns_handle *h, double *cflm)
     r.cflrow = h->cflrow;
     r.DX = r.DZ = 2.0 * PI / double(h->N);
     if ((st = big_launch(h, -1, r)) != DNS_OK) return st;
-    const size_t n = (size_t)(h->M / 2) * dns_big_cfl_tiles(h->M);
+    const size_t n = (size_t)dns_big_cfl_parts(h->M);
     std::vector<double> v(n);
     HIPCHK(h, hipMemcpyAsync(v.data(), h->cflrow, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -848,7 +848,7 @@ extern "C" dns_status dns_create(const dns_config *cfg, dns_handle **out)
             return bail(st);
     }
     if ((st = dev_alloc(h, (void **)&h->tw, sizeof(cd) * h->M, "twiddles")) != DNS_OK) return bail(st);
-    const size_t ncfl = h->big ? (size_t)(h->M / 2) * dns_big_cfl_tiles(h->M) : (size_t)h->M;
+    const size_t ncfl = h->big ? (size_t)dns_big_cfl_parts(h->M) : (size_t)h->M;
     if ((st = dev_alloc(h, (void **)&h->cflrow, sizeof(double) * ncfl, "cfl maxima")) != DNS_OK) return bail(st);
     if ((st = dev_alloc(h, (void **)&h->partEZ, sizeof(double) * 2 * h->N, "energy partials")) != DNS_OK)
         return bail(st);

@@ -24,7 +24,7 @@ enum {
     BIG_ZINV_B = 1,    // Y -> Z                                                 a = Y, b = Z
     BIG_XINV_A = 2,    // Z -> X (planes 1: u + i w; 2: also w_x + i w_z)       a = Z, b = X
     BIG_XINV_B = 3,    // X -> Yq (product, first forward step)                  a = X, b = Yq
-    BIG_XINV_CFL = 4,  // X (plane 0) -> cflrow[q * cfl_tiles + t] maxima        a = X
+    BIG_XINV_CFL = 4,  // X (plane 0) -> cflrow[workgroup] maxima               a = X
     BIG_XFWD_B = 5,    // Yq -> Qh                                               a = Yq, b = Qh
     BIG_ZFWD_A = 6,    // Qh -> W                                                a = Qh, b = W
     BIG_ZFWD_B = 7,    // W -> FN, CN/AB2 on om, fn                              a = W
@@ -84,6 +84,6 @@ struct BigReq {
 int dns_big_supported_m(int M);
 // 1 if the matrix-core line transforms (engine 1, 2) are built for line length M
 int dns_big_mfma_supported_m(int M);
-// CFL maxima per row pair written by BIG_XINV_CFL
-int dns_big_cfl_tiles(int M);
+// partial CFL maxima written by BIG_XINV_CFL (one per workgroup: two row pairs x C4 k1 lines)
+int dns_big_cfl_parts(int M);
 hipError_t dns_big_launch(const BigReq &r);

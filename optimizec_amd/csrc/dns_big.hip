@@ -460,10 +460,11 @@ __global__ __launch_bounds__(T) void k_xinv_a(const TC *__restrict__ Z, TC *__re
 }
 
 // ---- pass 4: x pass B of a row pair, product, first forward step -------------------------
-// row pairs per pass-4 workgroup: two in the u_i u_j form (the tables and barriers of a
-// workgroup then serve twice the lines), one otherwise
+// row pairs per pass-4 workgroup: two in the u_i u_j form and the CFL form (the tables and
+// barriers of a workgroup then serve twice the lines), one in the advective form (whose four
+// groups per pair already fill the LDS of three workgroups per CU)
 template <int M_, int MODE>
-constexpr int xb_pairs() { return MODE == 2 && M_ % 4 == 0 ? 2 : 1; }
+constexpr int xb_pairs() { return MODE != 0 && M_ % 4 == 0 ? 2 : 1; }
 // MODE 0: product -> Yq;  MODE 1: CFL maxima of u, w (plane 0 only) -> cflpart
 template <class TC, int M_, int MODE, int T>
 __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__restrict__ Yq, const TC *__restrict__ tw,
@@ -471,8 +472,9 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
 {
     using S = Split<M_>;
     constexpr int L1 = S::L1, L2 = S::L2, C = S::C4, LS = padded_len(L2);
-    // lines per k1: group g = 2 p + r (plane p, row 2q + r); MODE 2 (u_i u_j form) reads plane 0
-    // (u + i w) of RP row pairs (rows 2 RP q + g) and forward-transforms two products per pair
+    // lines per k1: group g = 2 p + r (plane p, row 2q + r); MODES 1 and 2 (CFL, u_i u_j form)
+    // read plane 0 (u + i w) of RP row pairs (rows 2 RP q + g); MODE 2 forward-transforms two
+    // products per pair
     constexpr int RP = xb_pairs<M_, MODE>();
     constexpr int G = MODE == 0 ? 4 : 2 * RP;
     __shared__ TC lds[G * C * LS];
@@ -490,7 +492,7 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
     for (int i = 0; i < IT; ++i) {
         const int e = threadIdx.x + i * T;
         const int n2 = e % L2, rest = e / L2, c = rest % C, g = rest / C;
-        const int p = MODE == 2 ? 0 : g >> 1, row = MODE == 2 ? 2 * RP * q + g : 2 * q + (g & 1);
+        const int p = MODE == 0 ? g >> 1 : 0, row = MODE == 0 ? 2 * q + (g & 1) : 2 * RP * q + g;
         xv[i] = e < CNT ? X[((size_t)p * M_ + row) * ms + (size_t)(k10 + c) * L2 + n2] : TC{0, 0};
     }
 #pragma unroll
@@ -507,7 +509,7 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
     fft_lines<L2, LS, G * C, T, true, kXBR, big_opq<TC, M_>(4)>(lds, w2, 1);
     if (MODE == 1) {
         double m = 0.0;
-        for (int e = threadIdx.x; e < 2 * C * L2; e += T) {
+        for (int e = threadIdx.x; e < G * C * L2; e += T) {
             const TC v = lds[(e / L2) * LS + pidx(e % L2)];
             const double cfl = fabs(double(v.x)) / DX + fabs(double(v.y)) / DZ;
             m = cfl > m ? cfl : m;
@@ -876,7 +878,7 @@ hipError_t launch_m(const BigReq &r)
                                    static_cast<const TC *>(r.a), static_cast<TC *>(r.b), tw, r.ms, nullptr, 0.0, 0.0);
             break;
         case BIG_XINV_CFL:
-            hipLaunchKernelGGL((k_xinv_b<TC, M_, 1, kTXB>), dim3(M_ / 2, S::L1 / S::C4), dim3(kTXB), 0, s,
+            hipLaunchKernelGGL((k_xinv_b<TC, M_, 1, kTXB>), dim3(M_ / (2 * xb_pairs<M_, 1>()), S::L1 / S::C4), dim3(kTXB), 0, s,
                                static_cast<const TC *>(r.a), nullptr, tw, r.ms, r.cflrow, r.DX, r.DZ);
             break;
         case BIG_XFWD_B:
@@ -945,8 +947,9 @@ hipError_t launch_lines(const BigReq &r, int M)
     return with_big_m(M, [&](auto m) { return launch_lines_m<decltype(m)::value>(r); });
 }
 
+// partial CFL maxima written by the CFL form of pass 4: one per workgroup
 template <int M_>
-constexpr int cfl_tiles_of() { return Split<M_>::L1 / Split<M_>::C4; }
+constexpr int cfl_parts_of() { return M_ / (2 * xb_pairs<M_, 1>()) * (Split<M_>::L1 / Split<M_>::C4); }
 
 }  // namespace
 
@@ -957,11 +960,11 @@ int dns_big_supported_m(int M)
 
 int dns_big_mfma_supported_m(int M) { return (M == 1536 || M == 44100) ? 1 : 0; }
 
-int dns_big_cfl_tiles(int M)
+int dns_big_cfl_parts(int M)
 {
     int t = 0;
     (void)with_big_m(M, [&](auto m) {
-        t = cfl_tiles_of<decltype(m)::value>();
+        t = cfl_parts_of<decltype(m)::value>();
         return hipSuccess;
     });
     return t;
