@@ -482,25 +482,41 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
     __shared__ double red[T / 64];
     using RT = decltype(TC::x);
     const int q = blockIdx.x, k10 = blockIdx.y * C;
-    constexpr int CNT = G * C * L2, IT = (CNT + T - 1) / T;
+    // TPN: one thread per line position n2 (L2 <= T < 2 L2, e.g. 210 points at 256 threads),
+    // every line's element n2 in that thread: the line, row and plane of each access are
+    // compile-time constants, where the flat element loop pays two divisions per element
+    constexpr bool TPN = L2 <= T && 2 * L2 > T;
+    constexpr int CNT = G * C * L2, IT = TPN ? G * C : (CNT + T - 1) / T;
     TabFetch<TC, L2, T> tf2;
     TabFetch<TC, L1, T> tfb;
     tf2.fetch(tw, L1);
     tfb.fetch(tw, 1);
     TC xv[IT];
+    const auto row_of = [&](int g) { return MODE == 0 ? (size_t)(g >> 1) * M_ + 2 * q + (g & 1) : (size_t)2 * RP * q + g; };
+    if constexpr (TPN) {
+        const int n2 = threadIdx.x;
+        if (n2 < L2) {
 #pragma unroll
-    for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * T;
-        const int n2 = e % L2, rest = e / L2, c = rest % C, g = rest / C;
-        const int p = MODE == 0 ? g >> 1 : 0, row = MODE == 0 ? 2 * q + (g & 1) : 2 * RP * q + g;
-        xv[i] = e < CNT ? X[((size_t)p * M_ + row) * ms + (size_t)(k10 + c) * L2 + n2] : TC{0, 0};
-    }
+            for (int g = 0; g < G; ++g)
 #pragma unroll
-    for (int i = 0; i < IT; ++i) {
-        const int e = threadIdx.x + i * T;
-        if (e < CNT) {
+                for (int c = 0; c < C; ++c) xv[g * C + c] = X[row_of(g) * ms + (size_t)(k10 + c) * L2 + n2];
+#pragma unroll
+            for (int l = 0; l < G * C; ++l) lds[l * LS + pidx(n2)] = xv[l];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int e = threadIdx.x + i * T;
             const int n2 = e % L2, rest = e / L2, c = rest % C, g = rest / C;
-            lds[(g * C + c) * LS + pidx(n2)] = xv[i];
+            xv[i] = e < CNT ? X[row_of(g) * ms + (size_t)(k10 + c) * L2 + n2] : TC{0, 0};
+        }
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int e = threadIdx.x + i * T;
+            if (e < CNT) {
+                const int n2 = e % L2, rest = e / L2, c = rest % C, g = rest / C;
+                lds[(g * C + c) * LS + pidx(n2)] = xv[i];
+            }
         }
     }
     tf2.put(w2);
@@ -531,21 +547,45 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
         // D = u^2 - w^2 and C = u w of rows 2j (real parts) and 2j + 1 (imaginary parts) of
         // pair j = RP q + pp: lines 2 pp C + c hold D, lines (2 pp + 1) C + c hold C; all
         // x-forwarded, written to Yq planes 0 (D) and 1 (C)
-        for (int e = threadIdx.x; e < RP * C * L2; e += T) {
-            const int pc = e / L2, k2 = e % L2, pp = pc / C, c = pc - pp * C;
-            const int l0 = 2 * pp * C + c, l1 = l0 + C;
+        const auto prod = [&](int l0, int k2) {
+            const int l1 = l0 + C;
             const TC a0 = lds[l0 * LS + pidx(k2)], a1 = lds[l1 * LS + pidx(k2)];
             lds[l0 * LS + pidx(k2)] = TC{RT(a0.x * a0.x - a0.y * a0.y), RT(a1.x * a1.x - a1.y * a1.y)};
             lds[l1 * LS + pidx(k2)] = TC{RT(a0.x * a0.y), RT(a1.x * a1.y)};
+        };
+        if constexpr (TPN) {
+            if (threadIdx.x < L2)
+#pragma unroll
+                for (int pp = 0; pp < RP; ++pp)
+#pragma unroll
+                    for (int c = 0; c < C; ++c) prod(2 * pp * C + c, threadIdx.x);
+        } else {
+            for (int e = threadIdx.x; e < RP * C * L2; e += T) {
+                const int pc = e / L2, k2 = e % L2, pp = pc / C, c = pc - pp * C;
+                prod(2 * pp * C + c, k2);
+            }
         }
         __syncthreads();
         fft_lines<L2, LS, 2 * RP * C, T, false, kXBR, big_opq<TC, M_>(4)>(lds, w2, 1);
-        for (int e = threadIdx.x; e < 2 * RP * C * L2; e += T) {
-            const int gc = e / L2, m2 = e % L2;
-            const int g = gc / C, c = gc - g * C, pp = g >> 1, pl = g & 1;
-            const int k1 = k10 + c;
-            Yq[((size_t)pl * (M_ / 2) + RP * q + pp) * ms + (size_t)k1 * L2 + m2] =
-                mul(lds[gc * LS + pidx(m2)], twid2<TC, L1, false>(w2, wb, k1 * m2));
+        // line g C + c: pair pp = g / 2, plane g mod 2 (D, C), k1 = k10 + c
+        const auto out = [&](int g, int c, int m2, TC w) {
+            Yq[((size_t)(g & 1) * (M_ / 2) + RP * q + (g >> 1)) * ms + (size_t)(k10 + c) * L2 + m2] =
+                mul(lds[(g * C + c) * LS + pidx(m2)], w);
+        };
+        if constexpr (TPN) {
+            const int m2 = threadIdx.x;
+            if (m2 < L2)
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const TC w = twid2<TC, L1, false>(w2, wb, (k10 + c) * m2);  // one twiddle for the 2 RP lines
+#pragma unroll
+                    for (int g = 0; g < 2 * RP; ++g) out(g, c, m2, w);
+                }
+        } else {
+            for (int e = threadIdx.x; e < 2 * RP * C * L2; e += T) {
+                const int gc = e / L2, m2 = e % L2, g = gc / C, c = gc - g * C;
+                out(g, c, m2, twid2<TC, L1, false>(w2, wb, (k10 + c) * m2));
+            }
         }
         return;
     }
