@@ -591,18 +591,30 @@ __global__ __launch_bounds__(T) void k_xinv_b(const TC *__restrict__ X, TC *__re
     }
     // P = u w_x + w w_z on rows 2q (real part) and 2q + 1 (imaginary part): group 0/1 hold
     // u + i w of the two rows, group 2/3 w_x + i w_z
-    for (int e = threadIdx.x; e < C * L2; e += T) {
-        const int c = e / L2, k2 = e % L2;
+    const auto prod = [&](int c, int k2) {
         const TC a0 = lds[(0 * C + c) * LS + pidx(k2)], a1 = lds[(1 * C + c) * LS + pidx(k2)];
         const TC b0 = lds[(2 * C + c) * LS + pidx(k2)], b1 = lds[(3 * C + c) * LS + pidx(k2)];
         lds[c * LS + pidx(k2)] = TC{RT(a0.x * b0.x + a0.y * b0.y), RT(a1.x * b1.x + a1.y * b1.y)};
+    };
+    const auto out = [&](int c, int m2) {
+        const int k1 = k10 + c;
+        Yq[(size_t)q * ms + (size_t)k1 * L2 + m2] = mul(lds[c * LS + pidx(m2)], twid2<TC, L1, false>(w2, wb, k1 * m2));
+    };
+    if constexpr (TPN) {
+        if (threadIdx.x < L2)
+#pragma unroll
+            for (int c = 0; c < C; ++c) prod(c, threadIdx.x);
+    } else {
+        for (int e = threadIdx.x; e < C * L2; e += T) prod(e / L2, e % L2);
     }
     __syncthreads();
     fft_lines<L2, LS, C, T, false, kXBR, big_opq<TC, M_>(4)>(lds, w2, 1);
-    for (int e = threadIdx.x; e < C * L2; e += T) {
-        const int c = e / L2, m2 = e % L2;
-        const int k1 = k10 + c;
-        Yq[(size_t)q * ms + (size_t)k1 * L2 + m2] = mul(lds[c * LS + pidx(m2)], twid2<TC, L1, false>(w2, wb, k1 * m2));
+    if constexpr (TPN) {
+        if (threadIdx.x < L2)
+#pragma unroll
+            for (int c = 0; c < C; ++c) out(c, threadIdx.x);
+    } else {
+        for (int e = threadIdx.x; e < C * L2; e += T) out(e / L2, e % L2);
     }
 }
 
